@@ -1,0 +1,184 @@
+#!/usr/bin/env python3
+"""Benchmark of the per-step hot path on one MI355X: SSPRK(3,3) steps of the forced turbulent
+channel at 4096x2049 FP64 (the reference solver's operator sequence, main.cu:434-541, with the
+adaptive dt chosen on the device).
+
+    python bench.py [--gpus 1] [--steps 500] [--warmup 200]
+
+Prints one JSON line: throughput in grid-point steps per second, the roofline of the dominant
+kernel (HIP events on the launch stream during the timed steps, algorithmic bytes per launch) and
+the CPU baseline (the FP64 OpenMP oracle timed on a bounded sample of the same workload on this
+host's cores).
+"""
+import argparse
+import ctypes
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "nse2d-channel-gpu_amd"))
+import nse2d_hip as nh  # noqa: E402
+
+METRIC = "grid-point·steps/s at 4096×2049 FP64 (% HBM roofline); LES speedup at ≤2% Re_τ err"
+UNIT = "grid-point·steps/s"
+HBM_PEAK_GBS = 8000.0
+
+# Forced turbulent channel, 4096 x 2049 FP64: Lx = 2 pi, h = 1, tanh beta = 2, Ub = 1, Re_b = 2e4,
+# Kolmogorov forcing F0 = 0.05 at n = 12, drag 1e-3, laminar profile + band-limited (kx <= 32)
+# seeded noise of amplitude 0.05 tapered by (1-y^2)^2, adaptive dt (CFL 0.5, viscous limit).
+CFG = dict(Nx=4096, Ny=2049, Lx=2 * math.pi, h=1.0, beta=2.0, Ub=1.0, Reb=2e4, F0=0.05, nforce=12, drag=1e-3,
+           amp=0.05, band_kmax=32, seed=20241203, cfl=0.5, cvisc=0.5, dt_max=1e-2)
+
+# Algorithmic HBM bytes per grid point for one launch of each kernel class (compulsory traffic,
+# FP64): stage kernel reads psi, omega_s (+ omega^n in stages 2 and 3) and writes omega_{s+1}
+# (24/32/32 B, mean 88/3); forward FFT reads the real row (8 B) and writes the half spectrum
+# (8 B); Thomas reads and writes the half spectrum (16 B); inverse FFT 16 B; CFL reads psi (8 B).
+ALG_BYTES = {"stage": 88.0 / 3.0, "fft_forward": 16.0, "thomas": 16.0, "fft_inverse": 16.0, "cfl": 8.0}
+
+
+def ic_args(c):
+    return ["--Nx", str(c["Nx"]), "--Ny", str(c["Ny"]), "--Lx", repr(c["Lx"]), "--h", repr(c["h"]),
+            "--Ub", repr(c["Ub"]), "--Reb", repr(c["Reb"]), "--init", "lamband", "--amp", repr(c["amp"]),
+            "--band_kmax", str(c["band_kmax"]), "--seed", str(c["seed"]), "--stretch", "tanh",
+            "--beta", repr(c["beta"]), "--F0", repr(c["F0"]), "--nforce", str(c["nforce"]),
+            "--drag", repr(c["drag"]), "--cfl", repr(c["cfl"]), "--cvisc", repr(c["cvisc"])]
+
+
+def cargv(args):
+    return (ctypes.c_char_p * len(args))(*[a.encode() for a in args])
+
+
+def setup_channel(c):
+    """Metrics and IC through the library's host code, state on the device."""
+    L = nh.lib()
+    Nx, Ny = c["Nx"], c["Ny"]
+    m = {k: np.zeros(Ny if k in ("y", "a_node", "w_node") else Ny - 1) for k in
+         ("y", "a_node", "a_edge", "dy_edge", "w_node")}
+    rc = L.nse2d_host_metrics(Ny, c["h"], b"tanh", c["beta"], *[m[k].ctypes.data for k in
+                              ("y", "a_node", "a_edge", "dy_edge", "w_node")])
+    if rc:
+        raise RuntimeError(nh.last_error())
+    w0 = np.zeros(Nx * Ny)
+    args = ic_args(c)
+    if L.nse2d_host_init_omega(len(args), cargv(args), w0.ctypes.data):
+        raise RuntimeError(nh.last_error())
+    dm = {k: nh.DeviceBuffer.from_array(v) for k, v in m.items()}
+    d = nh.ChannelDesc()
+    d.Nx, d.Ny, d.Lx, d.h = Nx, Ny, c["Lx"], c["h"]
+    d.nu = 2.0 * c["h"] * c["Ub"] / c["Reb"]
+    d.F0, d.lin_drag, d.nforce = c["F0"], c["drag"], c["nforce"]
+    d.psi_bot, d.psi_top = 0.0, 2.0 * c["h"] * c["Ub"]
+    d.cfl, d.cvisc, d.dt_max = c["cfl"], c["cvisc"], c["dt_max"]
+    d.dy_min = float(np.min(np.abs(m["dy_edge"])))
+    d.d_a_node, d.d_a_edge, d.d_y_node = dm["a_node"].ptr, dm["a_edge"].ptr, dm["y"].ptr
+    d.d_dy_edge, d.d_w_node = dm["dy_edge"].ptr, dm["w_node"].ptr
+    ch = L.nse2d_channel_create(ctypes.byref(d), None)
+    if not ch:
+        raise RuntimeError(nh.last_error())
+    nh.upload_ptr(L.nse2d_channel_field(ch, nh.FIELD_OMEGA), w0)
+    L.nse2d_channel_init(ch)
+    return ch, dm
+
+
+def read_profile(ch):
+    ms = (ctypes.c_double * 6)()
+    n = (ctypes.c_longlong * 6)()
+    nh.lib().nse2d_channel_profile_read(ch, ms, n)
+    return {nh.PROF_CLASSES[k]: (ms[k], n[k]) for k in range(6)}
+
+
+def cpu_baseline(c, budget_s):
+    """The oracle (FP64 OpenMP restatement) on this host: one warm step, then as many steps as fit
+    in about budget_s seconds (at least 1)."""
+    lib_path = os.path.join(ROOT, "oracle", "liboracle_nse2d.so")
+    if not os.path.isfile(lib_path):
+        return None
+    L = ctypes.CDLL(lib_path)
+    L.orc_time_steps.restype = ctypes.c_double
+    L.orc_time_steps.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), ctypes.c_int, ctypes.c_int]
+    L.orc_num_threads.restype = ctypes.c_int
+    args = ic_args(c)
+    t1 = L.orc_time_steps(len(args), cargv(args), 1, 1)  # includes one warm step
+    steps = int(max(1, min(20, budget_s / max(t1, 1e-3))))
+    el = L.orc_time_steps(len(args), cargv(args), 1, steps)
+    pts = c["Nx"] * c["Ny"] * steps
+    return {"value": pts / el, "unit": UNIT, "cores": int(L.orc_num_threads()), "kind": "port",
+            "sample": f"{steps} SSPRK(3,3) steps (adaptive dt) of the {c['Nx']}x{c['Ny']} workload after one "
+                      f"warm step, FP64 OpenMP oracle, {el:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported (the channel "
+              "would shard by y-slabs with a partitioned Thomas solve, which is not built)", file=sys.stderr)
+        sys.exit(2)
+    nh.require_gpu()
+    L = nh.lib()
+    c = CFG
+    Nx, Ny = c["Nx"], c["Ny"]
+    ch, keep = setup_channel(c)
+    # warm-up, then a short fully profiled segment to find the dominant kernel class
+    L.nse2d_channel_step_auto(ch, a.warmup)
+    L.nse2d_channel_profile(ch, (1 << 6) - 1)
+    L.nse2d_channel_step_auto(ch, 10)
+    prof = read_profile(ch)
+    per_step = {k: v[0] / 10.0 for k, v in prof.items()}
+    dom = max(ALG_BYTES, key=lambda k: prof[k][0])
+    dom_bit = nh.PROF_CLASSES.index(dom)
+    # timed region: only the dominant class is bracketed with events
+    L.nse2d_channel_profile(ch, 1 << dom_bit)
+    L.nse2d_device_sync()
+    t0 = time.perf_counter()
+    L.nse2d_channel_step_auto(ch, a.steps)
+    L.nse2d_device_sync()
+    t1 = time.perf_counter()
+    dom_ms, dom_n = read_profile(ch)[dom]
+    L.nse2d_channel_profile(ch, 0)
+    w = nh.download_ptr(L.nse2d_channel_field(ch, nh.FIELD_OMEGA), Nx * Ny)
+    if not np.all(np.isfinite(w)):
+        raise RuntimeError("non-finite vorticity after the timed steps")
+    t_sim = nh.download_ptr(L.nse2d_channel_time_ptr(ch), 1)[0]
+    elapsed = t1 - t0
+    value = Nx * Ny * a.steps / elapsed
+    avg_s = dom_ms / max(dom_n, 1) * 1e-3
+    bytes_per_launch = ALG_BYTES[dom] * Nx * Ny
+    achieved = bytes_per_launch / avg_s / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{dom}.json")
+    if os.path.isfile(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("bytes_per_launch")
+    out = {
+        "metric": METRIC, "value": value, "unit": UNIT, "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": elapsed / a.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "forced turbulent channel 4096x2049 FP64 (tanh beta=2, Re_b=2e4, F0=0.05 n=12, "
+                               "drag 1e-3, band-limited kx<=32 noise IC, adaptive dt CFL 0.5)",
+                   "Nx": Nx, "Ny": Ny, "sim_time_reached": t_sim},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "avg_launch_us": avg_s * 1e6, "alg_bytes_per_launch": bytes_per_launch},
+        "step_alg_bytes_per_point": 240,
+        "step_roofline_frac": 240.0 * value / (HBM_PEAK_GBS * 1e9),
+        "kernels_ms_per_step": per_step,
+    }
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(c, a.cpu_seconds)
+    L.nse2d_channel_destroy(ch)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,284 @@
+/*
+ * nse2d_hip.h - C-ABI of the MI355X (gfx950) implementation of the per-step hot path of the
+ * 2-D vorticity-streamfunction channel solver xupingxie/nse2d-channel-gpu.
+ *
+ * Conventions (the reference's, src/test_api.cu:1-92, src/ops.cuh, src/diag.cuh):
+ *   - every array argument is a DEVICE pointer; fields are FP64 row-major [Ny][Nx], x fastest
+ *     (common.cuh:127-130); metric vectors have Ny (a_node, y_node, w_node) or Ny-1 entries
+ *     (a_edge, dy_edge);
+ *   - all buffers are caller-owned, including reduction output scalars; the launchers write
+ *     their outputs completely (no memset needed by the caller);
+ *   - everything is asynchronous on the given stream; results are visible after a stream sync;
+ *   - launch and runtime failures abort the process with a message, like CUDA_CHECK
+ *     (common.cuh:116-119); argument errors that the host can see (unsupported sizes, missing
+ *     setup) are reported through a return value and nse2d_last_error();
+ *   - the reduction launchers use a library-owned device scratch and are not re-entrant across
+ *     host threads or concurrent streams of one device, like the reference's function-static
+ *     scratch (diag.cu:600-606, 843-855).
+ * Citations are file:line in the reference repository.
+ */
+#ifndef NSE2D_HIP_H
+#define NSE2D_HIP_H
+
+#include <stddef.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------------------------
+ * 1. The seven C-linkage test wrappers, signatures kept (cudaStream_t -> hipStream_t).
+ * ------------------------------------------------------------------------------------------ */
+/* test_api.cu:8-18 */
+void testapi_uv_from_psi_mapped(const double* d_psi, double* d_u, double* d_v, int Nx, int Ny,
+                                double inv_dx, double inv_deta, const double* d_a_node,
+                                hipStream_t s);
+/* test_api.cu:20-30 */
+void testapi_omega_from_psi_interior_mapped(double* d_w, const double* d_psi, int Nx, int Ny,
+                                            double inv_dx2, double inv_deta2,
+                                            const double* d_a_node, const double* d_a_edge,
+                                            hipStream_t s);
+/* test_api.cu:32-42 */
+void testapi_thom_wall_vorticity_mapped(double* d_w, const double* d_psi, int Nx, int Ny,
+                                        const double* d_dy_edge, double psi_bot,
+                                        double psi_top, hipStream_t s);
+/* test_api.cu:44-53 */
+void testapi_arakawa_J_mapped(const double* d_psi, const double* d_w, double* d_J, int Nx,
+                              int Ny, double inv_dx, double inv_deta, const double* d_a_node,
+                              hipStream_t s);
+/* test_api.cu:55-65 */
+void testapi_compute_adv_from_uv_mapped(const double* d_u, const double* d_v,
+                                        const double* d_omg, double* d_Jref, int Nx, int Ny,
+                                        double inv_dx, double inv_deta, const double* d_a_node,
+                                        hipStream_t s);
+/* test_api.cu:67-77 */
+void testapi_compute_adv_flux_mapped(const double* d_u, const double* d_v, const double* d_omg,
+                                     double* d_Jref, int Nx, int Ny, double inv_dx,
+                                     double inv_deta, const double* d_a_node, hipStream_t s);
+/* test_api.cu:79-92; any of J1/J2/J3/Jadv may be NULL */
+void testapi_arakawa_J_mapped_split(const double* d_psi, const double* d_omg, double* d_J1,
+                                    double* d_J2, double* d_J3, double* d_Jadv, int Nx, int Ny,
+                                    double inv_dx, double inv_deta, const double* d_a_node,
+                                    hipStream_t s);
+
+/* ------------------------------------------------------------------------------------------
+ * 2. Operator launchers (ops.cuh:268-337), same names and parameter order, C linkage.
+ * ------------------------------------------------------------------------------------------ */
+/* ops.cuh:268-269, ops.cu:64-70 */
+void launch_uv_from_psi_mapped(const double* psi, double* u, double* v, int Nx, int Ny,
+                               double inv_dx, double inv_deta, const double* d_a_node,
+                               hipStream_t s);
+/* ops.cuh:274-276, ops.cu:94-101; writes interior rows only */
+void launch_omega_from_psi_interior_mapped(double* w, const double* psi, int Nx, int Ny,
+                                           double inv_dx2, double inv_deta2,
+                                           const double* d_a_node, const double* d_a_edge,
+                                           hipStream_t s);
+/* ops.cuh:279-280, ops.cu:270-276; wall type from nse2d_set_wall_type() */
+void launch_thom_wall_vorticity_mapped(double* w, const double* psi, int Nx, int Ny,
+                                       const double* d_dy_edge, double psi_bot, double psi_top,
+                                       hipStream_t s);
+/* ops.cuh:287-288, ops.cu:328-334 */
+void launch_arakawa_J_mapped(const double* psi, const double* omg, double* J, int Nx, int Ny,
+                             double inv_dx, double inv_deta, const double* d_a_node,
+                             hipStream_t s);
+/* ops.cuh:312-316, ops.cu:501-512 */
+void launch_laplacian_fd_mapped(const double* d_q, double* d_out, int Nx, int Ny,
+                                double inv_dx2, double inv_deta2, const double* d_a_node,
+                                const double* d_a_edge, hipStream_t s);
+/* ops.cuh:322-325, ops.cu:536-544 */
+void launch_rhs_mapped(const double* J, const double* lap, const double* omg, double* rhs,
+                       int Nx, int Ny, double nu, double F0, int nforce, double h,
+                       double lin_drag, const double* d_y_node, hipStream_t s);
+/* ops.cuh:326-331, ops.cu:628-641 */
+void launch_mms_add_forcing(double* d_rhs, int Nx, int Ny, double Lx, double h, double t,
+                            double nu, double r, double A0, double eps, double Om, int kx_mode,
+                            const double* d_y_node, hipStream_t s);
+/* ops.cuh:335-337, ops.cu:656-669; launch_ssprk3 allows wnp1 == w (main.cu:530) */
+void launch_axpy(double* y, const double* x, const double* r, double dt, size_t N,
+                 hipStream_t s);
+void launch_ssprk2(double* w2, const double* w, const double* w1, const double* rhs, double dt,
+                   size_t N, hipStream_t s);
+void launch_ssprk3(double* wnp1, const double* w, const double* w2, const double* rhs,
+                   double dt, size_t N, hipStream_t s);
+
+/* Wall closure selection (no reference counterpart: the reference chooses at compile time,
+ * ops.cu:108-111 / Makefile WALLS=freeslip). 0 = no-slip Thom (default), 1 = free-slip. */
+void nse2d_set_wall_type(int free_slip);
+int nse2d_get_wall_type(void);
+
+/* ------------------------------------------------------------------------------------------
+ * 3. Poisson solver: x-FFT (hand-written LDS FFT, real data, half spectrum) + y-Thomas.
+ *    Replaces struct Poisson2D_Z2Z (poisson.cuh:18-119) and its cuFFT Z2Z plans.
+ *    Nx must be a power of two, 4 <= Nx <= 16384; Ny >= 4.
+ * ------------------------------------------------------------------------------------------ */
+typedef struct nse2d_poisson* nse2d_poisson_t;
+/* poisson.cuh:34-55 (constructor); NULL on unsupported sizes (see nse2d_last_error) */
+nse2d_poisson_t poisson2d_create(int Nx, int Ny, double Lx, hipStream_t s);
+/* poisson.cuh:71-80 (enable_mapped); borrows a_node/a_edge; returns 0 on success */
+int poisson2d_enable_mapped(nse2d_poisson_t p, const double* d_a_node, const double* d_a_edge,
+                            double inv_deta, hipStream_t s);
+/* poisson.cuh:83-118 (solve): -lap psi = omega, psi = psi_bot/psi_top on the walls.
+ * dy is unused (API compatibility, as in the reference). d_omega may not alias d_psi. */
+void poisson2d_solve(nse2d_poisson_t p, const double* d_omega, double* d_psi, double dy,
+                     double psi_bot, double psi_top, hipStream_t s);
+/* poisson.cuh:65-67 (set_profile / profile_reset / profile_get): event timing per phase */
+void poisson2d_set_profile(nse2d_poisson_t p, int on);
+void poisson2d_profile_reset(nse2d_poisson_t p);
+void poisson2d_profile_get(nse2d_poisson_t p, float* fft_ms, float* tridiag_ms);
+/* poisson.cuh:56-63 (destructor) */
+void poisson2d_destroy(nse2d_poisson_t p);
+
+/* ------------------------------------------------------------------------------------------
+ * 4. Diagnostics reductions (diag.cuh:12-113), same names and parameter order. Outputs are
+ *    the raw sums (the host applies 1/A, nu, 1/2 as main.cu:627-675 does).
+ * ------------------------------------------------------------------------------------------ */
+/* diag.cuh:12-14, diag.cu:6-38 (interior rows) */
+void launch_enstrophy_diss_mapped(const double* omg, int Nx, int Ny, double inv_dx,
+                                  double inv_deta, const double* d_a_node,
+                                  const double* d_w_node, double* d_sum_w2, double* d_sum_gw2,
+                                  hipStream_t s);
+/* diag.cuh:20-22, diag.cu:43-101 (all rows, one-sided eta derivative at the walls) */
+void launch_enstrophy_diss_mapped_full(const double* omg, int Nx, int Ny, double inv_dx,
+                                       double inv_deta, const double* d_a_node,
+                                       const double* d_w_node, double* d_sum_w2_full,
+                                       double* d_sum_gw2_full, hipStream_t s);
+/* diag.cuh:26-27, diag.cu:158-199 */
+void launch_enstrophy_sbp_mapped(const double* omg, int Nx, int Ny, double inv_dx,
+                                 double inv_deta, const double* d_a_edge,
+                                 const double* d_wt_node, double* d_sum_eta_h,
+                                 double* d_sum_phi_h, hipStream_t s);
+/* diag.cuh:31-32, diag.cu:201-231 */
+void launch_kinetic_energy_mapped(const double* psi, int Nx, int Ny, double inv_dx,
+                                  double inv_deta, const double* d_a_node,
+                                  const double* d_w_node, double* d_sum, hipStream_t s);
+/* diag.cuh:33-40, diag.cu:250-348 */
+void launch_energy_dissipation_mapped(const double* d_psi, const double* d_omega, int Nx,
+                                      int Ny, double inv_dx, double inv_deta,
+                                      const double* d_a_node, const double* d_a_edge,
+                                      const double* d_w_node, double* d_sum_eps,
+                                      hipStream_t s);
+/* diag.cuh:45-47, diag.cu:350-376 */
+void launch_power_input_mapped(const double* psi, int Nx, int Ny, double inv_deta, double h,
+                               double F0, int nforce, const double* d_a_node,
+                               const double* d_y_node, const double* d_w_node, double* d_sum,
+                               hipStream_t s);
+/* diag.cuh:54-57, diag.cu:411-422 */
+void launch_enstrophy_input_mapped_full(const double* omega, int Nx, int Ny, double h,
+                                        double F0, int nforce, const double* d_y_node,
+                                        const double* d_w_node, double* d_sum, hipStream_t s);
+/* diag.cuh:64-67, diag.cu:456-467 (unused by the reference driver; kept for completeness) */
+void launch_enstrophy_input_mapped(const double* omega, int Nx, int Ny, double h, double F0,
+                                   int nforce, const double* d_y_node, const double* d_w_node,
+                                   double* d_sum, hipStream_t s);
+/* diag.cuh:72-73, diag.cu:493-499 (per-row sums, Ny entries each) */
+void launch_row_moments_from_psi_mapped(const double* psi, int Nx, int Ny, double inv_dx,
+                                        double inv_deta, const double* d_a_node, double* su,
+                                        double* sv, double* su2, double* sv2, double* suv,
+                                        hipStream_t s);
+/* diag.cuh:75, diag.cu:508-512 (means over x of the two wall rows) */
+void launch_mean_wall_omega(const double* omega, int Nx, int Ny, double* d_mean_bot,
+                            double* d_mean_top, hipStream_t s);
+/* diag.cuh:83-88, diag.cu:571-583 */
+void launch_mean_wall_omega_omegay_mapped(const double* omega, int Nx, int Ny,
+                                          double inv_deta, const double* d_a_node,
+                                          double* d_mean_bot, double* d_mean_top,
+                                          hipStream_t s);
+/* diag.cuh:89-98, diag.cu:586-625 (unused by the reference driver; kept for completeness) */
+void launch_enstrophy_diss_balance_mapped(const double* d_omega, int Nx, int Ny,
+                                          double inv_dx2, double inv_deta2,
+                                          const double* d_a_node, const double* d_a_edge,
+                                          const double* d_w_node, double* d_sum_w2,
+                                          double* d_sum_eta, hipStream_t s);
+/* diag.cuh:99-105, diag.cu:672-691 */
+void launch_omega_rhs_balance_mapped(const double* d_omega, const double* d_rhs, int Nx,
+                                     int Ny, const double* d_w_node, double* d_sum_omega_rhs,
+                                     hipStream_t stream);
+/* diag.cuh:111-113, diag.cu:821-866 (max|u|, max|v|, max|v|*a over interior rows) */
+void launch_max_uv_from_psi_mapped(const double* d_psi, int Nx, int Ny, double inv_dx,
+                                   double inv_deta, const double* d_a_node, double* d_umax,
+                                   double* d_vmax, double* d_vmaxm, hipStream_t stream);
+
+/* ------------------------------------------------------------------------------------------
+ * 5. Whole-step entry points: the SSPRK(3,3) step of main.cu:482-538 with each stage's
+ *    stencil work (Jacobian, Laplacian, RHS, forcing, update) fused into one kernel, then the
+ *    Poisson solve and the Thom closure. No reference counterpart as one call; the sequence
+ *    and the arithmetic are main.cu's.
+ * ------------------------------------------------------------------------------------------ */
+typedef struct {
+    int Nx, Ny;
+    double Lx, h;
+    double nu, F0, lin_drag;
+    int nforce;
+    double psi_bot, psi_top;        /* main.cu:211: 0 and 2 h Ub */
+    int mms;                        /* add the manufactured forcing (ops.cu:559-626) */
+    double mms_A0, mms_eps, mms_Om;
+    int mms_kx;
+    /* adaptive-dt rule of main.cu:451-458, used by nse2d_channel_step_auto */
+    double cfl, cvisc, dt_max, dy_min;
+    /* borrowed device metric arrays */
+    const double *d_a_node, *d_a_edge, *d_y_node, *d_dy_edge, *d_w_node;
+} nse2d_channel_desc;
+
+typedef struct nse2d_channel* nse2d_channel_t;
+
+enum { NSE2D_FIELD_OMEGA = 0, NSE2D_FIELD_PSI = 1, NSE2D_FIELD_RHS = 2, NSE2D_FIELD_W1 = 3,
+       NSE2D_FIELD_W2 = 4 };
+enum { NSE2D_STEP_STORE_RHS = 1,     /* keep the stage-3 RHS in the RHS field (diag column) */
+       NSE2D_STEP_WALL_BOOKKEEPING = 2 /* accumulate the enstrophy jump of each Thom update */ };
+
+/* Allocates the state (omega, omega1, omega2, psi, rhs) and the Poisson plan. NULL on error. */
+nse2d_channel_t nse2d_channel_create(const nse2d_channel_desc* desc, hipStream_t s);
+void nse2d_channel_destroy(nse2d_channel_t ch);
+/* Device pointer of a state field ([Ny][Nx]); the caller may read or overwrite it. */
+double* nse2d_channel_field(nse2d_channel_t ch, int which);
+/* Initial solve + closure (main.cu:211-213): psi from omega, then omega's wall rows. */
+void nse2d_channel_init(nse2d_channel_t ch);
+/* One SSPRK(3,3) step with host-chosen dt (main.cu:482-538). flags: NSE2D_STEP_*. */
+void nse2d_channel_step(nse2d_channel_t ch, double t, double dt, int flags);
+/* n steps with dt chosen on the device by the rule of main.cu:440-458 (no host sync; the
+ * current dt and t live in device memory). t advances from its device value. */
+void nse2d_channel_step_auto(nse2d_channel_t ch, int n);
+/* Device-resident time and dt (read/write through these device pointers). */
+double* nse2d_channel_time_ptr(nse2d_channel_t ch);
+double* nse2d_channel_dt_ptr(nse2d_channel_t ch);
+/* Cumulative enstrophy added by the Thom update of each stage (main.cu:271-297), 3 values. */
+void nse2d_channel_wall_bookkeeping(nse2d_channel_t ch, double out3[3], double invA_full);
+/* Kernel-class timing with HIP events recorded on the channel's stream around the launches of
+ * the classes in class_mask (bit k = class k); resets the accumulators. Replaces the reference's
+ * --profile event timing (main.cu:303,436,540; poisson.cuh:88-117). */
+enum { NSE2D_PROF_STAGE = 0, NSE2D_PROF_FFT_FWD = 1, NSE2D_PROF_THOMAS = 2, NSE2D_PROF_FFT_INV = 3,
+       NSE2D_PROF_THOM = 4, NSE2D_PROF_CFL = 5, NSE2D_PROF_NCLASS = 6 };
+void nse2d_channel_profile(nse2d_channel_t ch, int class_mask);
+/* Synchronises the stream and returns the accumulated milliseconds and launch counts per class. */
+void nse2d_channel_profile_read(nse2d_channel_t ch, double* ms, long long* count);
+
+/* ------------------------------------------------------------------------------------------
+ * 6. Runtime helpers (device memory, streams) so that callers without another HIP binding
+ *    (ctypes, cgo) can drive the library.
+ * ------------------------------------------------------------------------------------------ */
+void* nse2d_dev_alloc(size_t bytes);
+void nse2d_dev_free(void* p);
+void nse2d_memcpy_h2d(void* dst, const void* src, size_t bytes);
+void nse2d_memcpy_d2h(void* dst, const void* src, size_t bytes);
+void nse2d_memcpy_d2d(void* dst, const void* src, size_t bytes);
+void nse2d_device_sync(void);
+int nse2d_device_count(void);
+const char* nse2d_last_error(void);
+const char* nse2d_version(void);
+
+/* Host-side setup with the reference's definitions: grid metrics (grid.cu:7-94; stretch "none",
+ * "uniform" or "tanh") and the initial vorticity (init.cu:99-378, plus the seeded benchmark ICs
+ * "lamnoise" and "lamband"); argv is the reference command line. Return 0 on success. */
+int nse2d_host_metrics(int Ny, double h, const char* stretch, double beta, double* y,
+                       double* a_node, double* a_edge, double* dy_edge, double* w_node);
+int nse2d_host_init_omega(int argc, char** argv, double* w_out);
+
+/* Host driver with the reference's command line (main.cu:13-816): runs a case, writing
+ * diagnostics.csv, profiles_timeavg.csv, progress.log and snapshots into --outdir. */
+int nse2d_main(int argc, char** argv);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NSE2D_HIP_H */

@@ -1,0 +1,165 @@
+"""Parity of the HIP Poisson solver (LDS FFT + Thomas) and of the diagnostics reductions against
+the oracle (poisson.cu / diag.cu restatements) on seeded inputs."""
+import math
+
+import numpy as np
+import pytest
+
+import nse2d_hip as nh
+import oracle
+from gpu_util import Grid, dev, rel_err, scalars, smooth_random_field, sync, zeros_dev
+from oracle import P
+
+pytestmark = pytest.mark.gpu
+
+POISSON_GRIDS = [(4, 4, 2.0), (8, 9, 0.0), (64, 33, 2.0), (128, 129, 2.0), (256, 65, 0.0), (512, 257, 2.0),
+                 (2048, 129, 2.0), (8192, 33, 2.0), (16384, 17, 2.5)]
+
+
+@pytest.mark.parametrize("Nx,Ny,beta", POISSON_GRIDS)
+def test_poisson_solve_matches_oracle(Nx, Ny, beta):
+    print(f"[poisson] {Nx}x{Ny} beta={beta}", flush=True)
+    g = Grid(Nx, Ny, beta)
+    w = smooth_random_field(Nx, Ny, 11, kmax=min(8, Nx // 2), scale=5.0, y=g.m.y)
+    for psi_bot, psi_top in ((0.0, 0.0), (0.0, 2.0)):
+        ref = oracle.poisson_solve(w, g.Lx, g.m, psi_bot, psi_top)
+        p = nh.lib().poisson2d_create(Nx, Ny, g.Lx, None)
+        assert p, nh.last_error()
+        assert nh.lib().poisson2d_enable_mapped(p, g.dp("a_node"), g.dp("a_edge"), g.inv_deta, None) == 0
+        dw, dpsi = dev(w), zeros_dev(Nx * Ny)
+        nh.lib().poisson2d_solve(p, dw.ptr, dpsi.ptr, 0.0, psi_bot, psi_top, None)
+        got = dpsi.download((Ny, Nx))
+        nh.lib().poisson2d_destroy(p)
+        assert rel_err(got, ref) <= 1e-13, (psi_bot, psi_top)
+        np.testing.assert_allclose(got[0], psi_bot, atol=1e-15)
+        np.testing.assert_allclose(got[-1], psi_top, atol=1e-14)
+
+
+def test_poisson_full_size_residual():
+    """4096x2049 (the benchmark grid): GPU solve vs oracle, and the discrete residual of
+    -(spectral d_xx + mapped d_yy) psi = omega on the interior rows (a property independent of
+    either implementation)."""
+    Nx, Ny = 4096, 2049
+    print(f"[poisson] full size {Nx}x{Ny}", flush=True)
+    g = Grid(Nx, Ny, 2.0)
+    w = smooth_random_field(Nx, Ny, 12, kmax=32, scale=5.0, y=g.m.y)
+    ref = oracle.poisson_solve(w, g.Lx, g.m, 0.0, 2.0)
+    p = nh.lib().poisson2d_create(Nx, Ny, g.Lx, None)
+    nh.lib().poisson2d_enable_mapped(p, g.dp("a_node"), g.dp("a_edge"), g.inv_deta, None)
+    dw, dpsi = dev(w), zeros_dev(Nx * Ny)
+    nh.lib().poisson2d_solve(p, dw.ptr, dpsi.ptr, 0.0, 0.0, 2.0, None)
+    got = dpsi.download((Ny, Nx))
+    nh.lib().poisson2d_destroy(p)
+    assert rel_err(got, ref) <= 1e-12
+    k = np.fft.fftfreq(Nx, d=1.0 / Nx)
+    k[Nx // 2] = Nx // 2
+    kx2 = (2 * math.pi * k / g.Lx) ** 2
+    dxx = np.real(np.fft.ifft(-kx2[None, :] * np.fft.fft(got, axis=1), axis=1))
+    a, ae = g.m.a_node, g.m.a_edge
+    dyy = a[1:-1, None] * (ae[1:, None] * (got[2:] - got[1:-1]) - ae[:-1, None] * (got[1:-1] - got[:-2])) * g.inv_deta2
+    res = dxx[1:-1] + dyy + w[1:-1]
+    assert np.max(np.abs(res)) <= 1e-8 * np.max(np.abs(w))
+
+
+def test_poisson_rejects_bad_sizes():
+    assert not nh.lib().poisson2d_create(48, 33, 1.0, None)
+    assert "power of two" in nh.last_error()
+    assert not nh.lib().poisson2d_create(64, 3, 1.0, None)
+
+
+DIAG_GRIDS = [(64, 33, 2.0), (512, 257, 2.0), (256, 129, 0.0), (96, 37, 2.5)]
+
+
+@pytest.fixture(scope="module", params=DIAG_GRIDS, ids=lambda g: f"{g[0]}x{g[1]}b{g[2]}")
+def dcase(request):
+    Nx, Ny, beta = request.param
+    print(f"[diag] grid {Nx}x{Ny} beta={beta}", flush=True)
+    g = Grid(Nx, Ny, beta)
+    psi = smooth_random_field(Nx, Ny, 21, y=g.m.y)
+    w = smooth_random_field(Nx, Ny, 22, scale=2.0, y=g.m.y)
+    rhs = smooth_random_field(Nx, Ny, 23, y=g.m.y)
+    return g, psi, w, rhs
+
+
+def _s(buf, k=0):
+    return float(buf.download()[k])
+
+
+def test_reductions_match_oracle(dcase):
+    g, psi, w, rhs = dcase
+    Nx, Ny, L = g.Nx, g.Ny, oracle.lib()
+    dpsi, dw, drhs = dev(psi), dev(w), dev(rhs)
+    out = scalars(2)
+    tol = 1e-13
+
+    def pair(orc_fn, hip_fn, orc_args, hip_args):
+        ref = np.zeros(2)
+        getattr(L, orc_fn)(*orc_args, P(ref))
+        getattr(nh.lib(), hip_fn)(*hip_args, out.ptr, out.ptr + 8, None)
+        got = out.download()
+        for a, b in zip(got, ref):
+            assert abs(a - b) <= tol * max(abs(b), 1e-300) or abs(a - b) < 1e-300, (hip_fn, a, b)
+
+    pair("orc_enstrophy_diss_mapped", "launch_enstrophy_diss_mapped",
+         (P(w), Nx, Ny, g.inv_dx, g.inv_deta, P(g.m.a_node), P(g.m.w_node)),
+         (dw.ptr, Nx, Ny, g.inv_dx, g.inv_deta, g.dp("a_node"), g.dp("w_node")))
+    pair("orc_enstrophy_diss_mapped_full", "launch_enstrophy_diss_mapped_full",
+         (P(w), Nx, Ny, g.inv_dx, g.inv_deta, P(g.m.a_node), P(g.m.w_node)),
+         (dw.ptr, Nx, Ny, g.inv_dx, g.inv_deta, g.dp("a_node"), g.dp("w_node")))
+    pair("orc_enstrophy_sbp_mapped", "launch_enstrophy_sbp_mapped",
+         (P(w), Nx, Ny, g.inv_dx, g.inv_deta, P(g.m.a_edge), P(g.m.wt_node)),
+         (dw.ptr, Nx, Ny, g.inv_dx, g.inv_deta, g.dp("a_edge"), g.dp("wt_node")))
+    pair("orc_enstrophy_diss_balance_mapped", "launch_enstrophy_diss_balance_mapped",
+         (P(w), Nx, Ny, g.inv_dx2, g.inv_deta2, P(g.m.a_node), P(g.m.a_edge), P(g.m.w_node)),
+         (dw.ptr, Nx, Ny, g.inv_dx2, g.inv_deta2, g.dp("a_node"), g.dp("a_edge"), g.dp("w_node")))
+    pair("orc_mean_wall_omega", "launch_mean_wall_omega", (P(w), Nx, Ny), (dw.ptr, Nx, Ny))
+    pair("orc_mean_wall_omega_omegay_mapped", "launch_mean_wall_omega_omegay_mapped",
+         (P(w), Nx, Ny, g.inv_deta, P(g.m.a_node)), (dw.ptr, Nx, Ny, g.inv_deta, g.dp("a_node")))
+
+    def single(orc_fn, hip_fn, orc_args, hip_args):
+        ref = getattr(L, orc_fn)(*orc_args)
+        getattr(nh.lib(), hip_fn)(*hip_args, out.ptr, None)
+        got = _s(out)
+        assert abs(got - ref) <= tol * abs(ref), (hip_fn, got, ref)
+
+    single("orc_kinetic_energy_mapped", "launch_kinetic_energy_mapped",
+           (P(psi), Nx, Ny, g.inv_dx, g.inv_deta, P(g.m.a_node), P(g.m.w_node)),
+           (dpsi.ptr, Nx, Ny, g.inv_dx, g.inv_deta, g.dp("a_node"), g.dp("w_node")))
+    single("orc_energy_dissipation_mapped", "launch_energy_dissipation_mapped",
+           (P(psi), P(w), Nx, Ny, g.inv_dx, g.inv_deta, P(g.m.a_node), P(g.m.a_edge), P(g.m.w_node)),
+           (dpsi.ptr, dw.ptr, Nx, Ny, g.inv_dx, g.inv_deta, g.dp("a_node"), g.dp("a_edge"), g.dp("w_node")))
+    single("orc_power_input_mapped", "launch_power_input_mapped",
+           (P(psi), Nx, Ny, g.inv_deta, 1.0, 0.05, 12, P(g.m.a_node), P(g.m.y), P(g.m.w_node)),
+           (dpsi.ptr, Nx, Ny, g.inv_deta, 1.0, 0.05, 12, g.dp("a_node"), g.dp("y"), g.dp("w_node")))
+    single("orc_enstrophy_input_mapped_full", "launch_enstrophy_input_mapped_full",
+           (P(w), Nx, Ny, 1.0, 0.05, 12, P(g.m.y), P(g.m.w_node)),
+           (dw.ptr, Nx, Ny, 1.0, 0.05, 12, g.dp("y"), g.dp("w_node")))
+    single("orc_enstrophy_input_mapped", "launch_enstrophy_input_mapped",
+           (P(w), Nx, Ny, 1.0, 0.05, 12, P(g.m.y), P(g.m.w_node)),
+           (dw.ptr, Nx, Ny, 1.0, 0.05, 12, g.dp("y"), g.dp("w_node")))
+    single("orc_omega_rhs_balance_mapped", "launch_omega_rhs_balance_mapped",
+           (P(w), P(rhs), Nx, Ny, P(g.m.w_node)), (dw.ptr, drhs.ptr, Nx, Ny, g.dp("w_node")))
+
+
+def test_max_uv_and_row_moments(dcase):
+    g, psi, w, rhs = dcase
+    Nx, Ny = g.Nx, g.Ny
+    dpsi = dev(psi)
+    ref = np.zeros(3)
+    oracle.lib().orc_max_uv_from_psi_mapped(P(psi), Nx, Ny, g.inv_dx, g.inv_deta, P(g.m.a_node), P(ref))
+    out = scalars(3)
+    nh.lib().launch_max_uv_from_psi_mapped(dpsi.ptr, Nx, Ny, g.inv_dx, g.inv_deta, g.dp("a_node"), out.ptr,
+                                           out.ptr + 8, out.ptr + 16, None)
+    got = out.download()
+    assert rel_err(got, ref) <= 1e-14
+    refs = [np.zeros(Ny) for _ in range(5)]
+    oracle.lib().orc_row_moments_from_psi_mapped(P(psi), Nx, Ny, g.inv_dx, g.inv_deta, P(g.m.a_node),
+                                                 *[P(r) for r in refs])
+    outs = [zeros_dev(Ny) for _ in range(5)]
+    nh.lib().launch_row_moments_from_psi_mapped(dpsi.ptr, Nx, Ny, g.inv_dx, g.inv_deta, g.dp("a_node"),
+                                                *[o.ptr for o in outs], None)
+    sync()
+    for o, r in zip(outs, refs):
+        got = o.download()
+        scale = max(np.max(np.abs(r)), 1e-300)
+        assert np.max(np.abs(got - r)) <= 1e-13 * scale * np.sqrt(Nx)

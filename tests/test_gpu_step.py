@@ -1,0 +1,172 @@
+"""The fused SSPRK(3,3) step (nse2d_channel_step: stage kernels + Poisson + Thom) against the
+oracle's unfused sequence of main.cu:482-538, for one step and after 1000 steps (bar 1e-10), and
+the device-resident dt path against the host dt rule."""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+import nse2d_hip as nh
+import oracle
+from gpu_util import Grid, rel_err, sync
+from oracle import P
+
+pytestmark = pytest.mark.gpu
+L2PI = 2 * math.pi
+
+
+class OracleStepper:
+    """The oracle's restatement of one SSPRK(3,3) step (main.cu:482-538)."""
+
+    def __init__(self, g, nu, F0, nforce, drag, psi_top, free_slip=0, mms=None):
+        self.g, self.nu, self.F0, self.nforce, self.drag = g, nu, F0, nforce, drag
+        self.psi_top, self.fs, self.mms = psi_top, free_slip, mms
+        n = (g.Ny, g.Nx)
+        self.J, self.lap, self.rhs = np.zeros(n), np.zeros(n), np.zeros(n)
+
+    def solve(self, w):
+        psi = oracle.poisson_solve(w, self.g.Lx, self.g.m, 0.0, self.psi_top)
+        oracle.lib().orc_thom_wall_vorticity_mapped(P(w), P(psi), self.g.Nx, self.g.Ny, P(self.g.m.dy_edge), 0.0,
+                                                    self.psi_top, self.fs)
+        return psi
+
+    def build_rhs(self, psi, w, t):
+        g, L = self.g, oracle.lib()
+        L.orc_arakawa_J_mapped(P(psi), P(w), P(self.J), g.Nx, g.Ny, g.inv_dx, g.inv_deta, P(g.m.a_node))
+        L.orc_laplacian_fd_mapped(P(w), P(self.lap), g.Nx, g.Ny, g.inv_dx2, g.inv_deta2, P(g.m.a_node), P(g.m.a_edge))
+        L.orc_rhs_mapped(P(self.J), P(self.lap), P(w), P(self.rhs), g.Nx, g.Ny, self.nu, self.F0, self.nforce, g.h,
+                         self.drag, P(g.m.y))
+        if self.mms:
+            L.orc_mms_add_forcing(P(self.rhs), g.Nx, g.Ny, g.Lx, g.h, t, self.nu, self.drag, *self.mms, P(g.m.y))
+        return self.rhs
+
+    def step(self, psi, w, t, dt):
+        N = w.size
+        L = oracle.lib()
+        w1, w2 = np.zeros_like(w), np.zeros_like(w)
+        r = self.build_rhs(psi, w, t)
+        L.orc_axpy(P(w1), P(w), P(r), dt, N)
+        psi = self.solve(w1)
+        r = self.build_rhs(psi, w1, t + dt)
+        L.orc_ssprk2(P(w2), P(w), P(w1), P(r), dt, N)
+        psi = self.solve(w2)
+        r = self.build_rhs(psi, w2, t + 0.5 * dt)
+        L.orc_ssprk3(P(w), P(w), P(w2), P(r), dt, N)
+        psi = self.solve(w)
+        return psi, w
+
+
+def make_channel(g, nu, F0, nforce, drag, psi_top, mms=None, cfl=0.5, cvisc=0.5, dt_max=1e-2):
+    d = nh.ChannelDesc()
+    d.Nx, d.Ny, d.Lx, d.h = g.Nx, g.Ny, g.Lx, g.h
+    d.nu, d.F0, d.lin_drag, d.nforce = nu, F0, drag, nforce
+    d.psi_bot, d.psi_top = 0.0, psi_top
+    if mms:
+        d.mms, (d.mms_A0, d.mms_eps, d.mms_Om, d.mms_kx) = 1, mms
+    d.cfl, d.cvisc, d.dt_max = cfl, cvisc, dt_max
+    d.dy_min = float(np.min(np.abs(g.m.dy_edge)))
+    d.d_a_node, d.d_a_edge, d.d_y_node = g.dp("a_node"), g.dp("a_edge"), g.dp("y")
+    d.d_dy_edge, d.d_w_node = g.dp("dy_edge"), g.dp("w_node")
+    ch = nh.lib().nse2d_channel_create(ctypes.byref(d), None)
+    assert ch, nh.last_error()
+    return ch
+
+
+def field(ch, which, g):
+    return nh.download_ptr(nh.lib().nse2d_channel_field(ch, which), g.Nx * g.Ny, (g.Ny, g.Nx))
+
+
+def channel_ic(Nx, Ny, Lx=L2PI):
+    args = ["--Nx", str(Nx), "--Ny", str(Ny), "--Lx", repr(Lx), "--init", "lamband", "--amp", "0.05",
+            "--band_kmax", str(min(32, Nx // 4)), "--stretch", "tanh", "--beta", "2", "--Ub", "1", "--seed", "7"]
+    return oracle.init_omega(args, Nx, Ny)
+
+
+@pytest.mark.parametrize("Nx,Ny,mms", [(64, 33, None), (256, 129, None), (128, 65, (1.0, 0.1, 1.0, 2))])
+def test_one_step_matches_oracle(Nx, Ny, mms):
+    print(f"[step] {Nx}x{Ny} mms={mms}", flush=True)
+    g = Grid(Nx, Ny, 2.0)
+    nu, F0, nf, drag = 1e-3, 0.05, 12, 1e-3
+    w0 = channel_ic(Nx, Ny)
+    o = OracleStepper(g, nu, F0, nf, drag, 2.0, mms=mms)
+    w_o = w0.copy()
+    psi_o = o.solve(w_o)
+    ch = make_channel(g, nu, F0, nf, drag, 2.0, mms=mms)
+    nh.upload_ptr(nh.lib().nse2d_channel_field(ch, nh.FIELD_OMEGA), w0)
+    nh.lib().nse2d_channel_init(ch)
+    sync()
+    assert rel_err(field(ch, nh.FIELD_PSI, g), psi_o) <= 1e-13
+    assert rel_err(field(ch, nh.FIELD_OMEGA, g), w_o) <= 1e-13
+    t, dt = 0.3, 1e-3
+    psi_o, w_o = o.step(psi_o, w_o, t, dt)
+    nh.lib().nse2d_channel_step(ch, t, dt, nh.STEP_STORE_RHS)
+    sync()
+    assert rel_err(field(ch, nh.FIELD_OMEGA, g), w_o) <= 1e-13
+    assert rel_err(field(ch, nh.FIELD_PSI, g), psi_o) <= 1e-13
+    assert rel_err(field(ch, nh.FIELD_RHS, g), o.rhs) <= 1e-13
+    nh.lib().nse2d_channel_destroy(ch)
+
+
+def test_thousand_steps_within_1e10():
+    """1000 steps on a 256x129 stretched channel (forcing, drag, noise IC): GPU and oracle stay
+    within 1e-10 normwise."""
+    Nx, Ny = 256, 129
+    g = Grid(Nx, Ny, 2.0)
+    nu, F0, nf, drag, dt = 1e-4, 0.05, 12, 1e-3, 2e-3
+    w0 = channel_ic(Nx, Ny)
+    o = OracleStepper(g, nu, F0, nf, drag, 2.0)
+    w_o = w0.copy()
+    psi_o = o.solve(w_o)
+    ch = make_channel(g, nu, F0, nf, drag, 2.0)
+    nh.upload_ptr(nh.lib().nse2d_channel_field(ch, nh.FIELD_OMEGA), w0)
+    nh.lib().nse2d_channel_init(ch)
+    t = 0.0
+    for k in range(1000):
+        psi_o, w_o = o.step(psi_o, w_o, t, dt)
+        nh.lib().nse2d_channel_step(ch, t, dt, 0)
+        t += dt
+        if k % 250 == 249:
+            print(f"[1000 steps] step {k + 1}", flush=True)
+    sync()
+    e_w = rel_err(field(ch, nh.FIELD_OMEGA, g), w_o)
+    e_p = rel_err(field(ch, nh.FIELD_PSI, g), psi_o)
+    print(f"[1000 steps] omega {e_w:.2e} psi {e_p:.2e}", flush=True)
+    assert e_w <= 1e-10 and e_p <= 1e-10
+    nh.lib().nse2d_channel_destroy(ch)
+
+
+def test_device_dt_matches_host_rule():
+    """nse2d_channel_step_auto chooses dt on the device with main.cu:451-458's rule; the host rule
+    on the oracle's maxima gives the same dt sequence and fields."""
+    Nx, Ny = 128, 65
+    g = Grid(Nx, Ny, 2.0)
+    nu, F0, nf, drag = 2e-4, 0.05, 12, 1e-3
+    cfl, cvisc, dt_max = 0.5, 0.5, 1e-2
+    w0 = channel_ic(Nx, Ny)
+    o = OracleStepper(g, nu, F0, nf, drag, 2.0)
+    w_o = w0.copy()
+    psi_o = o.solve(w_o)
+    ch = make_channel(g, nu, F0, nf, drag, 2.0, cfl=cfl, cvisc=cvisc, dt_max=dt_max)
+    nh.upload_ptr(nh.lib().nse2d_channel_field(ch, nh.FIELD_OMEGA), w0)
+    nh.lib().nse2d_channel_init(ch)
+    dy_min = float(np.min(np.abs(g.m.dy_edge)))
+    deta = 2.0 / (Ny - 1)
+    t = 0.0
+    for k in range(20):
+        m = np.zeros(3)
+        oracle.lib().orc_max_uv_from_psi_mapped(P(psi_o), Nx, Ny, g.inv_dx, g.inv_deta, P(g.m.a_node), P(m))
+        dt_x = cfl * (g.dx / m[0]) if m[0] > 0 else 1e30
+        dt_y = cfl * (deta / m[2]) if m[2] > 0 else 1e30
+        dt_vis = cvisc * 0.5 / (nu * (g.inv_dx2 + 1.0 / dy_min ** 2))
+        dt = min(dt_max, min(min(dt_x, dt_y), dt_vis))
+        psi_o, w_o = o.step(psi_o, w_o, t, dt)
+        t += dt
+        nh.lib().nse2d_channel_step_auto(ch, 1)
+        sync()
+        dt_dev = nh.download_ptr(nh.lib().nse2d_channel_dt_ptr(ch), 1)[0]
+        assert abs(dt_dev - dt) <= 1e-15 * dt, (k, dt_dev, dt)
+    t_dev = nh.download_ptr(nh.lib().nse2d_channel_time_ptr(ch), 1)[0]
+    assert abs(t_dev - t) <= 1e-14
+    assert rel_err(field(ch, nh.FIELD_OMEGA, g), w_o) <= 1e-12
+    nh.lib().nse2d_channel_destroy(ch)
