@@ -34,41 +34,68 @@ struct StageArgs {
     MmsParams mp;
 };
 
-constexpr int SX = 64, SY = 4;
+constexpr int SX = 64, SY = 4, RY = 8;
 
-// One thread per grid point. Interior rows: J and the Laplacian from the 3x3 neighbourhoods of
-// psi and omega_s; wall rows: J = lap = 0 (ops.cu:295, 483). The RHS is main.cu's
-// -J + nu lap + f(y) - r omega (+ manufactured forcing), then the SSPRK combination:
+// One thread per column x and RY consecutive rows, with a register window over rows j-1, j, j+1
+// of psi and omega_s (columns x-1, x, x+1): each row is loaded once per thread. Interior rows:
+// J and the Laplacian from the 3x3 neighbourhoods; wall rows: J = lap = 0 (ops.cu:295, 483).
+// The RHS is main.cu's -J + nu lap + f(y) - r omega (+ manufactured forcing), then the SSPRK
+// combination (ops.cu:644-655):
 //   stage 1: out = ws + dt R;  stage 2: out = 3/4 wn + 1/4 (ws + dt R);
-//   stage 3: out = 1/3 wn + 2/3 (ws + dt R)     (ops.cu:644-655)
+//   stage 3: out = 1/3 wn + 2/3 (ws + dt R)   (out may alias wn: each point reads then writes
+//   its own element of wn only)
 template <int STAGE>
 __global__ void __launch_bounds__(256) stage_kernel(StageArgs a) {
-    const int ix = blockIdx.x * SX + threadIdx.x, iy = blockIdx.y * SY + threadIdx.y;
-    if (ix >= a.Nx || iy >= a.Ny) return;
-    const int Nx = a.Nx;
-    const size_t c = idx2(iy, ix, Nx);
-    double J = 0.0, lap = 0.0, wc;
-    if (iy > 0 && iy < a.Ny - 1) {
-        const Stencil9 P = Stencil9::load(a.psi, Nx, ix, iy), W = Stencil9::load(a.ws, Nx, ix, iy);
-        const double an = a.a_node[iy];
-        double J1, J2, J3;
-        arakawa_terms(P, W, an, 0.25 * a.inv_dx * a.inv_deta, J1, J2, J3);
-        J = -((J1 + J2 + J3) * (1.0 / 3.0));
-        lap = mapped_laplacian(W.xm, W.c, W.xp, W.ym, W.yp, an, a.a_edge[iy], a.a_edge[iy - 1], a.inv_dx2,
-                               a.inv_deta2);
-        wc = W.c;
-    } else {
-        wc = a.ws[c];
-    }
-    double rhs = -J + a.nu * lap + a.f_row[iy] - a.lin_drag * wc;
-    if (a.mms) rhs += mms_f_omega(a.mp, ix, a.y_node[iy]);
-    if (a.rhs_out) a.rhs_out[c] = rhs;
+    const int Nx = a.Nx, Ny = a.Ny;
+    const int ix = blockIdx.x * SX + threadIdx.x;
+    const int jlo = (blockIdx.y * SY + threadIdx.y) * RY;
+    if (ix >= Nx || jlo >= Ny) return;
+    const int jhi = min(jlo + RY, Ny);
+    const int xm = wrap_x(ix - 1, Nx), xp = wrap_x(ix + 1, Nx);
+    const double inv4dxde = 0.25 * a.inv_dx * a.inv_deta;
     const double dt = a.dt_ptr ? *a.dt_ptr : a.dt_val;
-    double o;
-    if (STAGE == 1) o = wc + dt * rhs;
-    else if (STAGE == 2) o = 0.75 * a.wn[c] + 0.25 * (wc + dt * rhs);
-    else o = (1.0 / 3.0) * a.wn[c] + (2.0 / 3.0) * (wc + dt * rhs);
-    a.out[c] = o;
+    Stencil9 P, W;  // (m, c, p) of rows j-1 (mm, ym, pm), j (xm, c, xp), j+1 (mp, yp, pp)
+    auto row3 = [&](const double* __restrict__ f, int j, double& m, double& c, double& p) {
+        const double* r = f + (size_t)j * Nx;
+        m = r[xm]; c = r[ix]; p = r[xp];
+    };
+    if (jlo > 0) {
+        row3(a.psi, jlo - 1, P.mm, P.ym, P.pm);
+        row3(a.ws, jlo - 1, W.mm, W.ym, W.pm);
+    }
+    row3(a.psi, jlo, P.xm, P.c, P.xp);
+    row3(a.ws, jlo, W.xm, W.c, W.xp);
+#pragma unroll
+    for (int q = 0; q < RY; ++q) {
+        const int j = jlo + q;
+        if (j >= jhi) break;
+        if (j + 1 < Ny) {
+            row3(a.psi, j + 1, P.mp, P.yp, P.pp);
+            row3(a.ws, j + 1, W.mp, W.yp, W.pp);
+        }
+        double J = 0.0, lap = 0.0;
+        if (j > 0 && j < Ny - 1) {
+            const double an = a.a_node[j];
+            double J1, J2, J3;
+            arakawa_terms(P, W, an, inv4dxde, J1, J2, J3);
+            J = -((J1 + J2 + J3) * (1.0 / 3.0));
+            lap = mapped_laplacian(W.xm, W.c, W.xp, W.ym, W.yp, an, a.a_edge[j], a.a_edge[j - 1], a.inv_dx2,
+                                   a.inv_deta2);
+        }
+        const double wc = W.c;
+        double rhs = -J + a.nu * lap + a.f_row[j] - a.lin_drag * wc;
+        if (a.mms) rhs += mms_f_omega(a.mp, ix, a.y_node[j]);
+        const size_t c = idx2(j, ix, Nx);
+        if (a.rhs_out) a.rhs_out[c] = rhs;
+        double o;
+        if (STAGE == 1) o = wc + dt * rhs;
+        else if (STAGE == 2) o = 0.75 * a.wn[c] + 0.25 * (wc + dt * rhs);
+        else o = (1.0 / 3.0) * a.wn[c] + (2.0 / 3.0) * (wc + dt * rhs);
+        a.out[c] = o;
+        // slide the window down one row
+        P.mm = P.xm; P.ym = P.c; P.pm = P.xp; P.xm = P.mp; P.c = P.yp; P.xp = P.pp;
+        W.mm = W.xm; W.ym = W.c; W.pm = W.xp; W.xm = W.mp; W.c = W.yp; W.xp = W.pp;
+    }
 }
 
 // Forcing profile f(y_j) (ops.cu:526-531), evaluated once per row.
@@ -185,7 +212,7 @@ StageArgs stage_args(ChannelImpl* c, const double* ws, const double* wn, double*
 template <int STAGE>
 void launch_stage(ChannelImpl* c, const StageArgs& a) {
     ProfScope ps(c, NSE2D_PROF_STAGE);
-    const dim3 grid(ceil_div(c->d.Nx, SX), ceil_div(c->d.Ny, SY));
+    const dim3 grid(ceil_div(c->d.Nx, SX), ceil_div(c->d.Ny, SY * RY));
     stage_kernel<STAGE><<<grid, dim3(SX, SY), 0, c->s>>>(a);
     NSE2D_LAUNCH_CHECK();
 }

@@ -4,16 +4,24 @@
 //
 // Spectral layout: for each row j the Nx real values are transformed as one complex FFT of
 // M = Nx/2 points (even samples in the real part, odd samples in the imaginary part), then
-// split into the half spectrum X[0..M]. X[0] and X[M] are real for real data, so they share
-// one complex slot: hat[j][0] = (X[0], X[M]). A row of the spectrum is therefore exactly M
-// complex values, [Ny][M] row-major, kx fastest: the Thomas sweep runs one lane per kx and
-// every lane load of a row is coalesced. Lane 0 solves two real systems (kx = 0 with the wall
-// values, kx = Nyquist without).
+// split into the half spectrum X[0..M]. X[0] and X[M] are real for real data, so they share one
+// complex slot: hat[j][0] = (X[0], X[M]). A spectrum row is exactly M complex values, [Ny][M]
+// row-major with kx fastest, so that the y-direction solve reads and writes whole row segments;
+// slot 0 carries two real systems (kx = 0 with the wall values, kx = Nyquist without).
+//
+// y-direction solve (chunked Thomas): a workgroup owns KC kx columns and splits the Ny-2
+// interior rows into NCH chunks of RC rows, one lane per (column, chunk). Each lane runs the
+// forward elimination of its chunk with zero carry-in, keeping the local values in registers
+// and the chunk's affine transfer maps; an LDS scan composes the maps into the true carries;
+// the lanes then finish the back substitution and write the solution. HBM traffic is one read
+// and one write of the spectrum; the elimination coefficients are recomputed from the values
+// at the chunk boundaries (precomputed once per plan).
 #include "nse2d_common.hpp"
 #include "nse2d_poisson.hpp"
 #include "../../include/nse2d_hip.h"
 
 #include <cmath>
+#include <cstdlib>
 #include <string>
 #include <vector>
 
@@ -99,8 +107,7 @@ __device__ void lds_fft(cplx* __restrict__ buf, int M, int log2M, const cplx* __
 // Forward real FFT of each row of omega ([Ny][Nx]) into the packed half spectrum ([Ny][M]).
 template <int IT4>
 __global__ void __launch_bounds__(1024) fft_rows_forward_kernel(const double* __restrict__ omega,
-                                                               cplx* __restrict__ hat, int Nx,
-                                                               int log2M,
+                                                               cplx* __restrict__ hat, int Nx, int log2M,
                                                                const cplx* __restrict__ tw) {
     extern __shared__ cplx lds[];
     const int M = Nx >> 1, j = blockIdx.x, T = blockDim.x;
@@ -127,8 +134,7 @@ __global__ void __launch_bounds__(1024) fft_rows_forward_kernel(const double* __
 // Inverse: packed half spectrum row -> real row, scaled by `scale` (1/Nx).
 template <int IT4>
 __global__ void __launch_bounds__(1024) fft_rows_inverse_kernel(const cplx* __restrict__ hat,
-                                                               double* __restrict__ psi, int Nx,
-                                                               int log2M,
+                                                               double* __restrict__ psi, int Nx, int log2M,
                                                                const cplx* __restrict__ tw,
                                                                double scale) {
     extern __shared__ cplx lds[];
@@ -152,27 +158,8 @@ __global__ void __launch_bounds__(1024) fft_rows_inverse_kernel(const cplx* __re
     for (int n = threadIdx.x; n < M; n += T) dst[n] = cscale(lds[n], scale);
 }
 
-// poisson.cu:38-68 restated per column k = 0..M (M+1 columns, column M = Nyquist), written
-// transposed: cp[r][k] (row r = j-1 for j = 1..Ny-2).
-__global__ void precompute_cp_kernel(double* __restrict__ cp, const double* __restrict__ a_sub,
-                                     const double* __restrict__ c_sup,
-                                     const double* __restrict__ kx2, int M, int Ni) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k > M) return;
-    const int ld = M + 1;
-    double b = -(a_sub[0] + c_sup[0]) - kx2[k];
-    double cp_prev = c_sup[0] / b;
-    cp[k] = cp_prev;
-    for (int r = 1; r < Ni; ++r) {
-        b = -(a_sub[r] + c_sup[r]) - kx2[k];
-        const double den = b - a_sub[r] * cp_prev;
-        const double cpi = c_sup[r] / den;
-        cp[(size_t)r * ld + k] = cpi;
-        cp_prev = cpi;
-    }
-}
-
-// Row coefficients of the mapped operator, poisson.cu:83-88,103-107.
+// Row coefficients of the mapped operator, poisson.cu:83-88,103-107:
+// a_sub = a_j a_{j-1/2} / deta^2, c_sup = a_j a_{j+1/2} / deta^2 for interior row r = j-1.
 __global__ void row_coeff_kernel(double* __restrict__ a_sub, double* __restrict__ c_sup,
                                  const double* __restrict__ a_node,
                                  const double* __restrict__ a_edge, int Ny, double inv_deta2) {
@@ -183,52 +170,183 @@ __global__ void row_coeff_kernel(double* __restrict__ a_sub, double* __restrict_
     c_sup[r] = a_node[j] * a_edge[j] * inv_deta2;
 }
 
-// poisson.cu:70-133, one lane per packed column k, rows j = 1..Ny-2, in place on hat.
-// The right-hand side is -omega_hat; the wall values enter the kx = 0 system only, scaled by
-// Nx (unnormalised transform). Forward sweep divides by den = c_sup / cp like the reference.
-__global__ void __launch_bounds__(64) thomas_kernel(cplx* __restrict__ hat,
-                                                    const double* __restrict__ cp,
-                                                    const double* __restrict__ a_sub,
-                                                    const double* __restrict__ c_sup,
-                                                    const double* __restrict__ kx2, int M, int Ny,
-                                                    double bc_bot, double bc_top) {
+// Elimination recurrence of poisson.cu:38-68 (cp_r = c_r / (b_r - a_r cp_{r-1}), b_r =
+// -(a_r + c_r) - kx^2), one thread per kx (k = 0..M, k = M the Nyquist), storing the value that
+// enters each chunk: cpb[c][k] = cp_{c*RC - 1} for c >= 1. Same operations as the solve kernel,
+// so the chunked solve reproduces the serial recurrence exactly.
+__global__ void chunk_boundary_cp_kernel(double* __restrict__ cpb, const double* __restrict__ a_sub,
+                                         const double* __restrict__ c_sup, const double* __restrict__ kx2,
+                                         int ncols, int Ni, int RC, int NCH) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= M) return;
-    const int Ni = Ny - 2, ld = M + 1;
-    // real part uses column kr, imaginary part column ki (different only for lane 0)
-    const int kr = k, ki = (k == 0 ? M : k);
-    cplx* col = hat + k;
-    cplx dp_prev;
-    {
-        const double br = -(a_sub[0] + c_sup[0]) - kx2[kr];
-        const double bi = -(a_sub[0] + c_sup[0]) - kx2[ki];
-        cplx rhs = col[(size_t)M * 1];
-        rhs.x = -rhs.x; rhs.y = -rhs.y;
-        if (k == 0) rhs.x -= a_sub[0] * bc_bot;
-        dp_prev = {(rhs.x - a_sub[0] * 0.0) / br, (rhs.y - a_sub[0] * 0.0) / bi};
-        col[(size_t)M * 1] = dp_prev;
+    if (k >= ncols) return;
+    double cp_prev = 0.0;
+    for (int c = 0; c < NCH; ++c) {
+        if (c > 0) cpb[(size_t)c * ncols + k] = cp_prev;
+        for (int i = 0; i < RC; ++i) {
+            const int r = c * RC + i;
+            if (r >= Ni) break;
+            const double a = a_sub[r], cc = c_sup[r];
+            const double b = -(a + cc) - kx2[k];
+            const double gi = 1.0 / (b - a * cp_prev);
+            cp_prev = cc * gi;
+        }
     }
-    for (int r = 1; r < Ni; ++r) {
-        const int j = r + 1;
-        const double as = a_sub[r], cs = c_sup[r];
-        cplx rhs = col[(size_t)M * j];
-        rhs.x = -rhs.x; rhs.y = -rhs.y;
-        if (k == 0 && r == Ni - 1) rhs.x -= cs * bc_top;
-        const double den_r = cs / cp[(size_t)r * ld + kr];
-        const double den_i = cs / cp[(size_t)r * ld + ki];
-        dp_prev = {(rhs.x - as * dp_prev.x) / den_r, (rhs.y - as * dp_prev.y) / den_i};
-        col[(size_t)M * j] = dp_prev;
+}
+
+// g_r = 1 / (b_r - a_r cp_{r-1}) of one column (the Nyquist system of slot 0), all rows.
+__global__ void column_g_kernel(double* __restrict__ gcol, const double* __restrict__ a_sub,
+                                const double* __restrict__ c_sup, double kk, int Ni) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double cp_prev = 0.0;
+    for (int r = 0; r < Ni; ++r) {
+        const double a = a_sub[r], cc = c_sup[r];
+        const double gi = 1.0 / ((-(a + cc) - kk) - a * cp_prev);
+        gcol[r] = gi;
+        cp_prev = cc * gi;
     }
-    cplx xnext = col[(size_t)M * Ni];
-    for (int r = Ni - 2; r >= 0; --r) {
-        const int j = r + 1;
-        const cplx dp = col[(size_t)M * j];
-        const cplx xj = {dp.x - cp[(size_t)r * ld + kr] * xnext.x, dp.y - cp[(size_t)r * ld + ki] * xnext.y};
-        col[(size_t)M * j] = xj;
-        xnext = xj;
+}
+
+// Chunked Thomas solve (see the header comment), in place on hat. For interior row r (grid row
+// r+1) and spectral slot k: a_r x_{r-1} + b_r x_r + c_r x_{r+1} = -omega_hat, with the wall values
+// entering the kx = 0 system (scaled by Nx, unnormalised transform; poisson.cu:93-114) and
+// psi_hat = Nx psi_wall at kx = 0, 0 otherwise on the wall rows (poisson.cu:131-132).
+//   forward: dp_r = (f_r - a_r dp_{r-1}) g_r, g_r = 1 / (b_r - a_r cp_{r-1}), cp_r = c_r g_r
+//   backward: x_r = dp_r - cp_r x_{r+1}
+// Per chunk [s, e]: dp_r = dpL_r + P_r D with P_r = prod_{s..r} (-a_i g_i), D = dp_{s-1};
+// x_s = S + RP D + QS X with X = x_{e+1}, S = sum_r W_r dpL_r, RP = sum_r W_r P_r,
+// W_r = prod_{s..r-1} (-cp_i), QS = prod_{s..e} (-cp_i).
+// Real and imaginary parts carry their own coefficient chains; they differ only in slot 0 (real:
+// kx = 0, imaginary: Nyquist), whose second chain reads its precomputed g (gny).
+template <int RC, int KC, int T>
+__global__ void __launch_bounds__(T) thomas_chunked_kernel(cplx* __restrict__ hat, int M, int Ni,
+                                                           const double* __restrict__ a_sub,
+                                                           const double* __restrict__ c_sup,
+                                                           const double* __restrict__ kx2,
+                                                           const double* __restrict__ cpb,
+                                                           const double* __restrict__ gny, double bc_bot,
+                                                           double bc_top) {
+    constexpr int NCH = T / KC;
+    __shared__ double sm[2][4][T];
+    const int t = threadIdx.x, kc = t % KC, c = t / KC;
+    const int k = blockIdx.x * KC + kc;
+    const bool colok = k < M;
+    const bool split = (k == 0);
+    const int ldc = M + 1;
+    const int r0 = c * RC;
+    const int nr = colok ? max(0, min(RC, Ni - r0)) : 0;
+    const double kkx = colok ? kx2[k] : 0.0;
+
+    cplx f[RC];
+    double g[RC];
+#pragma unroll
+    for (int i = 0; i < RC; ++i)
+        if (i < nr) f[i] = hat[(size_t)(r0 + i + 1) * M + k];
+    double cpx = (c > 0 && nr > 0) ? cpb[(size_t)c * ldc + k] : 0.0;
+    cplx dpl = {0.0, 0.0}, S = {0.0, 0.0};
+    double Wx = 1.0, Wy = 1.0, Px = 1.0, Py = 1.0, RPx = 0.0, RPy = 0.0;
+#pragma unroll
+    for (int i = 0; i < RC; ++i) {
+        if (i < nr) {
+            const int r = r0 + i;
+            const double a = a_sub[r], cc = c_sup[r];
+            double rx = -f[i].x;
+            const double ry = -f[i].y;
+            if (split && r == 0) rx -= a * bc_bot;
+            if (split && r == Ni - 1) rx -= cc * bc_top;
+            const double gx = 1.0 / ((-(a + cc) - kkx) - a * cpx);
+            const double gy = split ? gny[r] : gx;
+            const double cpy = cc * gy;
+            cpx = cc * gx;
+            dpl = {(rx - a * dpl.x) * gx, (ry - a * dpl.y) * gy};
+            f[i] = dpl;
+            g[i] = gx;
+            Px *= -a * gx;
+            Py *= -a * gy;
+            S = {S.x + Wx * dpl.x, S.y + Wy * dpl.y};
+            RPx += Wx * Px;
+            RPy += Wy * Py;
+            Wx *= -cpx;
+            Wy *= -cpy;
+        }
     }
-    col[0] = (k == 0 ? cplx{bc_bot, 0.0} : cplx{0.0, 0.0});
-    col[(size_t)M * (Ny - 1)] = (k == 0 ? cplx{bc_top, 0.0} : cplx{0.0, 0.0});
+    // forward carries: inclusive scan over chunks of the maps D -> E + P D (per component)
+    int buf = 0;
+    sm[0][0][t] = dpl.x;
+    sm[0][1][t] = dpl.y;
+    sm[0][2][t] = Px;  // nr == 0: the identity (0, 1)
+    sm[0][3][t] = Py;
+    __syncthreads();
+#pragma unroll
+    for (int d = 1; d < NCH; d <<= 1) {
+        double ex = sm[buf][0][t], ey = sm[buf][1][t], px = sm[buf][2][t], py = sm[buf][3][t];
+        if (c >= d) {
+            const int u = t - d * KC;
+            ex += px * sm[buf][0][u];
+            ey += py * sm[buf][1][u];
+            px *= sm[buf][2][u];
+            py *= sm[buf][3][u];
+        }
+        sm[buf ^ 1][0][t] = ex;
+        sm[buf ^ 1][1][t] = ey;
+        sm[buf ^ 1][2][t] = px;
+        sm[buf ^ 1][3][t] = py;
+        buf ^= 1;
+        __syncthreads();
+    }
+    const cplx D = (c == 0) ? cplx{0.0, 0.0} : cplx{sm[buf][0][t - KC], sm[buf][1][t - KC]};
+    __syncthreads();
+    // backward carries: suffix scan of the maps X -> h + Q X, h = S + RP D
+    buf = 0;
+    sm[0][0][t] = S.x + RPx * D.x;
+    sm[0][1][t] = S.y + RPy * D.y;
+    sm[0][2][t] = Wx;
+    sm[0][3][t] = Wy;
+    __syncthreads();
+#pragma unroll
+    for (int d = 1; d < NCH; d <<= 1) {
+        double hx = sm[buf][0][t], hy = sm[buf][1][t], qx = sm[buf][2][t], qy = sm[buf][3][t];
+        if (c + d < NCH) {
+            const int u = t + d * KC;
+            hx += qx * sm[buf][0][u];
+            hy += qy * sm[buf][1][u];
+            qx *= sm[buf][2][u];
+            qy *= sm[buf][3][u];
+        }
+        sm[buf ^ 1][0][t] = hx;
+        sm[buf ^ 1][1][t] = hy;
+        sm[buf ^ 1][2][t] = qx;
+        sm[buf ^ 1][3][t] = qy;
+        buf ^= 1;
+        __syncthreads();
+    }
+    const cplx X = (c == NCH - 1) ? cplx{0.0, 0.0} : cplx{sm[buf][0][t + KC], sm[buf][1][t + KC]};
+    // true dp, then back substitution from x_{e+1} = X
+    double Qx = 1.0, Qy = 1.0;
+#pragma unroll
+    for (int i = 0; i < RC; ++i) {
+        if (i < nr) {
+            const double a = a_sub[r0 + i];
+            const double gy = split ? gny[r0 + i] : g[i];
+            Qx *= -a * g[i];
+            Qy *= -a * gy;
+            f[i] = {f[i].x + Qx * D.x, f[i].y + Qy * D.y};
+        }
+    }
+    cplx xn = X;
+#pragma unroll
+    for (int i = RC - 1; i >= 0; --i) {
+        if (i < nr) {
+            const int r = r0 + i;
+            const double cc = c_sup[r];
+            const double gy = split ? gny[r] : g[i];
+            xn = {f[i].x - (cc * g[i]) * xn.x, f[i].y - (cc * gy) * xn.y};
+            hat[(size_t)(r + 1) * M + k] = xn;
+        }
+    }
+    if (c == 0 && colok) {
+        hat[k] = (split ? cplx{bc_bot, 0.0} : cplx{0.0, 0.0});
+        hat[(size_t)(Ni + 1) * M + k] = (split ? cplx{bc_top, 0.0} : cplx{0.0, 0.0});
+    }
 }
 
 int ilog2(int n) {
@@ -241,30 +359,66 @@ int ilog2(int n) {
 
 struct PoissonImpl {
     int Nx = 0, Ny = 0, M = 0, log2M = 0, threads = 256, it4 = 1;
+    int rc = 1, kc = 8, nch = 64;  // chunked Thomas layout
     double Lx = 0, inv_deta2 = 0;
     bool mapped = false, profile = false;
     float acc_fft_ms = 0.f, acc_tridiag_ms = 0.f;
-    cplx* d_tw = nullptr;   // [Nx] exp(-2 pi i q / Nx)
-    double* d_kx2 = nullptr;  // [M+1]
-    double* d_cp = nullptr;   // [Ny-2][M+1]
+    cplx* d_tw = nullptr;      // [Nx] exp(-2 pi i q / Nx)
+    double* d_kx2 = nullptr;   // [M+1]
+    double* d_cpb = nullptr;   // [NCH][M+1] elimination coefficient entering each chunk
+    double* d_gny = nullptr;   // [Ny-2] g of the Nyquist system (slot 0, imaginary part)
     double* d_asub = nullptr;  // [Ny-2]
     double* d_csup = nullptr;  // [Ny-2]
-    cplx* d_hat = nullptr;    // [Ny][M]
+    cplx* d_hat = nullptr;     // [Ny][M]
     size_t lds_bytes = 0;
 };
+
+namespace {
+constexpr int kThomasThreads = 512;
+// Chunk geometry of the y solve: KC columns per workgroup, NCH = 512/KC chunks of RC rows
+// (RC a power of two, NCH * RC >= Ny-2). Instantiated: KC = 8 with RC <= 32, KC = 4 with
+// RC 16/32, KC = 2 with RC 8/16/32. The default takes the widest KC; NSE2D_THOMAS_KC selects
+// another column width (used to tune on the device).
+bool thomas_instantiated(int kc, int rc) {
+    if (kc == 8) return rc >= 1 && rc <= 32;
+    if (kc == 4) return rc == 16 || rc == 32;
+    if (kc == 2) return rc == 8 || rc == 16 || rc == 32;
+    return false;
+}
+void choose_thomas_layout(PoissonImpl* p) {
+    const int Ni = p->Ny - 2;
+    int pref = 8;
+    if (const char* e = std::getenv("NSE2D_THOMAS_KC")) pref = std::atoi(e);
+    for (int kc : {pref, 8, 4, 2}) {
+        if (kc != 8 && kc != 4 && kc != 2) continue;
+        const int nch = kThomasThreads / kc;
+        int rc = 1;
+        while (rc * nch < Ni) rc <<= 1;
+        while (rc <= 32 && !thomas_instantiated(kc, rc)) rc <<= 1;
+        if (rc <= 32) { p->kc = kc; p->nch = nch; p->rc = rc; return; }
+    }
+    p->kc = 0;
+}
+}  // namespace
 
 PoissonImpl* poisson_create(int Nx, int Ny, double Lx, hipStream_t s) {
     if (Nx < 4 || Nx > 16384 || (Nx & (Nx - 1)) != 0) {
         set_last_error("poisson2d_create: Nx must be a power of two in [4, 16384], got " + std::to_string(Nx));
         return nullptr;
     }
-    if (Ny < 4) {
-        set_last_error("poisson2d_create: Ny must be >= 4, got " + std::to_string(Ny));
+    if (Ny < 4 || Ny > 16386) {
+        set_last_error("poisson2d_create: Ny must be in [4, 16386], got " + std::to_string(Ny));
         return nullptr;
     }
     auto* p = new PoissonImpl;
     p->Nx = Nx; p->Ny = Ny; p->Lx = Lx;
     p->M = Nx / 2;
+    choose_thomas_layout(p);
+    if (p->kc == 0) {
+        set_last_error("poisson2d_create: Ny = " + std::to_string(Ny) + " exceeds the y-solve layouts");
+        delete p;
+        return nullptr;
+    }
     p->log2M = ilog2(p->M);
     p->lds_bytes = (size_t)p->M * sizeof(cplx);
     int dev = 0, max_lds = 0;
@@ -300,7 +454,8 @@ PoissonImpl* poisson_create(int Nx, int Ny, double Lx, hipStream_t s) {
     }
     NSE2D_HIP_CHECK(hipMalloc(&p->d_tw, sizeof(cplx) * Nx));
     NSE2D_HIP_CHECK(hipMalloc(&p->d_kx2, sizeof(double) * (p->M + 1)));
-    NSE2D_HIP_CHECK(hipMalloc(&p->d_cp, sizeof(double) * (size_t)(Ny - 2) * (p->M + 1)));
+    NSE2D_HIP_CHECK(hipMalloc(&p->d_cpb, sizeof(double) * (size_t)p->nch * (p->M + 1)));
+    NSE2D_HIP_CHECK(hipMalloc(&p->d_gny, sizeof(double) * (size_t)(Ny - 2)));
     NSE2D_HIP_CHECK(hipMalloc(&p->d_asub, sizeof(double) * (Ny - 2)));
     NSE2D_HIP_CHECK(hipMalloc(&p->d_csup, sizeof(double) * (Ny - 2)));
     NSE2D_HIP_CHECK(hipMalloc(&p->d_hat, sizeof(cplx) * (size_t)Ny * p->M));
@@ -317,8 +472,14 @@ int poisson_enable_mapped(PoissonImpl* p, const double* a_node, const double* a_
     const int Ni = p->Ny - 2;
     row_coeff_kernel<<<ceil_div(Ni, 256), 256, 0, s>>>(p->d_asub, p->d_csup, a_node, a_edge, p->Ny, p->inv_deta2);
     NSE2D_LAUNCH_CHECK();
-    precompute_cp_kernel<<<ceil_div(p->M + 1, 64), 64, 0, s>>>(p->d_cp, p->d_asub, p->d_csup, p->d_kx2, p->M, Ni);
+    chunk_boundary_cp_kernel<<<ceil_div(p->M + 1, 64), 64, 0, s>>>(p->d_cpb, p->d_asub, p->d_csup, p->d_kx2,
+                                                                  p->M + 1, Ni, p->rc, p->nch);
     NSE2D_LAUNCH_CHECK();
+    {
+        const double kny = (2.0 * kPi * (double)p->M) / p->Lx;
+        column_g_kernel<<<1, 64, 0, s>>>(p->d_gny, p->d_asub, p->d_csup, kny * kny, Ni);
+        NSE2D_LAUNCH_CHECK();
+    }
     p->mapped = true;
     return 0;
 }
@@ -331,8 +492,26 @@ void poisson_forward(PoissonImpl* p, const double* omega, hipStream_t s) {
     NSE2D_LAUNCH_CHECK();
 }
 void poisson_tridiag(PoissonImpl* p, double psi_bot, double psi_top, hipStream_t s) {
-    thomas_kernel<<<ceil_div(p->M, 64), 64, 0, s>>>(p->d_hat, p->d_cp, p->d_asub, p->d_csup, p->d_kx2, p->M, p->Ny,
-                                                   (double)p->Nx * psi_bot, (double)p->Nx * psi_top);
+    const int Ni = p->Ny - 2;
+    const double bb = (double)p->Nx * psi_bot, bt = (double)p->Nx * psi_top;
+    const unsigned grid = ceil_div(p->M, p->kc);
+#define NSE2D_THOMAS(RC, KC)                                                                              \
+    thomas_chunked_kernel<RC, KC, kThomasThreads><<<grid, kThomasThreads, 0, s>>>(                        \
+        p->d_hat, p->M, Ni, p->d_asub, p->d_csup, p->d_kx2, p->d_cpb, p->d_gny, bb, bt)
+    switch (p->kc * 100 + p->rc) {
+        case 801: NSE2D_THOMAS(1, 8); break;
+        case 802: NSE2D_THOMAS(2, 8); break;
+        case 804: NSE2D_THOMAS(4, 8); break;
+        case 808: NSE2D_THOMAS(8, 8); break;
+        case 816: NSE2D_THOMAS(16, 8); break;
+        case 832: NSE2D_THOMAS(32, 8); break;
+        case 416: NSE2D_THOMAS(16, 4); break;
+        case 432: NSE2D_THOMAS(32, 4); break;
+        case 208: NSE2D_THOMAS(8, 2); break;
+        case 216: NSE2D_THOMAS(16, 2); break;
+        default: NSE2D_THOMAS(32, 2); break;
+    }
+#undef NSE2D_THOMAS
     NSE2D_LAUNCH_CHECK();
 }
 void poisson_inverse(PoissonImpl* p, double* psi, hipStream_t s) {
@@ -377,7 +556,7 @@ void poisson_solve(PoissonImpl* p, const double* omega, double* psi, double psi_
 
 void poisson_destroy(PoissonImpl* p) {
     if (!p) return;
-    (void)hipFree(p->d_tw); (void)hipFree(p->d_kx2); (void)hipFree(p->d_cp);
+    (void)hipFree(p->d_tw); (void)hipFree(p->d_kx2); (void)hipFree(p->d_cpb); (void)hipFree(p->d_gny);
     (void)hipFree(p->d_asub); (void)hipFree(p->d_csup); (void)hipFree(p->d_hat);
     delete p;
 }

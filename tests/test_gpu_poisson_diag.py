@@ -30,7 +30,9 @@ def test_poisson_solve_matches_oracle(Nx, Ny, beta):
         nh.lib().poisson2d_solve(p, dw.ptr, dpsi.ptr, 0.0, psi_bot, psi_top, None)
         got = dpsi.download((Ny, Nx))
         nh.lib().poisson2d_destroy(p)
-        assert rel_err(got, ref) <= 1e-13, (psi_bot, psi_top)
+        e = rel_err(got, ref)
+        print(f"[poisson] {Nx}x{Ny} psi_top={psi_top} rel err {e:.2e}", flush=True)
+        assert e <= 1e-13, (psi_bot, psi_top)
         np.testing.assert_allclose(got[0], psi_bot, atol=1e-15)
         np.testing.assert_allclose(got[-1], psi_top, atol=1e-14)
 
@@ -38,7 +40,10 @@ def test_poisson_solve_matches_oracle(Nx, Ny, beta):
 def test_poisson_full_size_residual():
     """4096x2049 (the benchmark grid): GPU solve vs oracle, and the discrete residual of
     -(spectral d_xx + mapped d_yy) psi = omega on the interior rows (a property independent of
-    either implementation)."""
+    either implementation). At 2047 interior rows the serial FP64 elimination of the reference
+    differs from an extended-precision solve by up to ~5e-13 for the kx ~ 0 modes
+    (test_oracle_math.py::test_thomas_rounding_floor_at_2049_rows), so two FP64 solves are
+    compared at 5e-13 here; below 1000 rows the bar is 1e-13."""
     Nx, Ny = 4096, 2049
     print(f"[poisson] full size {Nx}x{Ny}", flush=True)
     g = Grid(Nx, Ny, 2.0)
@@ -50,7 +55,9 @@ def test_poisson_full_size_residual():
     nh.lib().poisson2d_solve(p, dw.ptr, dpsi.ptr, 0.0, 0.0, 2.0, None)
     got = dpsi.download((Ny, Nx))
     nh.lib().poisson2d_destroy(p)
-    assert rel_err(got, ref) <= 1e-12
+    e = rel_err(got, ref)
+    print(f"[poisson] full size rel err {e:.2e}", flush=True)
+    assert e <= 5e-13
     k = np.fft.fftfreq(Nx, d=1.0 / Nx)
     k[Nx // 2] = Nx // 2
     kx2 = (2 * math.pi * k / g.Lx) ** 2
@@ -85,20 +92,54 @@ def _s(buf, k=0):
     return float(buf.download()[k])
 
 
+def _abs_scales(g, psi, w, rhs):
+    """Sum of |terms| of the signed reductions (error bound scale of any summation order)."""
+    Nx, Ny, L = g.Nx, g.Ny, oracle.lib()
+    u, v = np.zeros_like(psi), np.zeros_like(psi)
+    L.orc_uv_from_psi_mapped(P(psi), P(u), P(v), Nx, Ny, g.inv_dx, g.inv_deta, P(g.m.a_node))
+    y, h = g.m.y, 1.0
+    Fx = 0.05 * np.sin(12 * math.pi * ((y + h) / (2 * h)))
+    fw = 0.05 * (12 * math.pi / (2 * h)) * np.abs(np.cos(12 * math.pi * ((y + h) / (2 * h))))
+    lap = np.zeros_like(w)
+    L.orc_laplacian_fd_mapped(P(w), P(lap), Nx, Ny, g.inv_dx2, g.inv_deta2, P(g.m.a_node), P(g.m.a_edge))
+    wn = g.m.w_node
+    return {
+        "launch_power_input_mapped": float(np.sum(wn[:, None] * np.abs(u * Fx[:, None]))),
+        "launch_enstrophy_input_mapped_full": float(np.sum(wn[:, None] * np.abs(w) * fw[:, None])),
+        "launch_enstrophy_input_mapped": float(np.sum(wn[1:-1, None] * np.abs(w[1:-1]) * fw[1:-1, None])),
+        "launch_omega_rhs_balance_mapped": float(np.sum(wn[:, None] * np.abs(w * rhs))),
+        "launch_enstrophy_diss_balance_mapped": float(np.sum(wn[1:-1, None] * (w[1:-1] ** 2 + np.abs(w[1:-1] * lap[1:-1])))),
+        "launch_mean_wall_omega": float(np.mean(np.abs(w[0])) + np.mean(np.abs(w[-1]))),
+        "launch_mean_wall_omega_omegay_mapped": float(
+            np.mean(np.abs(w[0]) * g.m.a_node[0] * (3 * np.abs(w[0]) + 4 * np.abs(w[1]) + np.abs(w[2])) * 0.5 * g.inv_deta)
+            + np.mean(np.abs(w[-1]) * g.m.a_node[-1] * (3 * np.abs(w[-1]) + 4 * np.abs(w[-2]) + np.abs(w[-3])) * 0.5 * g.inv_deta)),
+        "launch_enstrophy_sbp_mapped": float(
+            np.sum(g.m.wt_node[:, None] * (np.roll(w, -1, axis=1) - w) ** 2) * g.inv_dx2
+            + np.sum(g.m.a_edge[:, None] * np.diff(w, axis=0) ** 2) * g.inv_deta
+            + (np.sum(np.abs(w[0] * (w[1] - w[0]))) * g.m.a_edge[0] + np.sum(np.abs(w[-1] * (w[-1] - w[-2]))) * g.m.a_edge[-1]) * g.inv_deta),
+    }
+
+
 def test_reductions_match_oracle(dcase):
+    """Positive sums: relative error <= 1e-13. Signed sums: |error| <= 1e-13 * sum |terms|
+    (the bound of a sum evaluated in any order)."""
     g, psi, w, rhs = dcase
     Nx, Ny, L = g.Nx, g.Ny, oracle.lib()
     dpsi, dw, drhs = dev(psi), dev(w), dev(rhs)
     out = scalars(2)
     tol = 1e-13
+    scales = _abs_scales(g, psi, w, rhs)
+
+    def ok(fn, got, ref):
+        bound = tol * max(abs(ref), scales.get(fn, 0.0))
+        assert abs(got - ref) <= bound or abs(got - ref) < 1e-300, (fn, got, ref, bound)
 
     def pair(orc_fn, hip_fn, orc_args, hip_args):
         ref = np.zeros(2)
         getattr(L, orc_fn)(*orc_args, P(ref))
         getattr(nh.lib(), hip_fn)(*hip_args, out.ptr, out.ptr + 8, None)
-        got = out.download()
-        for a, b in zip(got, ref):
-            assert abs(a - b) <= tol * max(abs(b), 1e-300) or abs(a - b) < 1e-300, (hip_fn, a, b)
+        for a, b in zip(out.download(), ref):
+            ok(hip_fn, a, b)
 
     pair("orc_enstrophy_diss_mapped", "launch_enstrophy_diss_mapped",
          (P(w), Nx, Ny, g.inv_dx, g.inv_deta, P(g.m.a_node), P(g.m.w_node)),
@@ -119,8 +160,7 @@ def test_reductions_match_oracle(dcase):
     def single(orc_fn, hip_fn, orc_args, hip_args):
         ref = getattr(L, orc_fn)(*orc_args)
         getattr(nh.lib(), hip_fn)(*hip_args, out.ptr, None)
-        got = _s(out)
-        assert abs(got - ref) <= tol * abs(ref), (hip_fn, got, ref)
+        ok(hip_fn, _s(out), ref)
 
     single("orc_kinetic_energy_mapped", "launch_kinetic_energy_mapped",
            (P(psi), Nx, Ny, g.inv_dx, g.inv_deta, P(g.m.a_node), P(g.m.w_node)),
@@ -159,7 +199,9 @@ def test_max_uv_and_row_moments(dcase):
     nh.lib().launch_row_moments_from_psi_mapped(dpsi.ptr, Nx, Ny, g.inv_dx, g.inv_deta, g.dp("a_node"),
                                                 *[o.ptr for o in outs], None)
     sync()
-    for o, r in zip(outs, refs):
-        got = o.download()
-        scale = max(np.max(np.abs(r)), 1e-300)
-        assert np.max(np.abs(got - r)) <= 1e-13 * scale * np.sqrt(Nx)
+    # per-row sums of signed terms: bound by the row sums of |terms| (u, v as the kernel forms them)
+    u, v = np.zeros_like(psi), np.zeros_like(psi)
+    oracle.lib().orc_uv_from_psi_mapped(P(psi), P(u), P(v), Nx, Ny, g.inv_dx, g.inv_deta, P(g.m.a_node))
+    abs_rows = [np.sum(np.abs(t), axis=1) for t in (u, v, u * u, v * v, u * v)]
+    for o, r, a in zip(outs, refs, abs_rows):
+        assert np.all(np.abs(o.download() - r) <= 1e-13 * a + 1e-300)

@@ -1,6 +1,13 @@
 """The fused SSPRK(3,3) step (nse2d_channel_step: stage kernels + Poisson + Thom) against the
 oracle's unfused sequence of main.cu:482-538, for one step and after 1000 steps (bar 1e-10), and
-the device-resident dt path against the host dt rule."""
+the device-resident dt path against the host dt rule.
+
+A step composes three stage updates, three Poisson solves and three Thom closures, each checked
+on its own at 1e-13 (test_gpu_ops, test_gpu_poisson_diag). The closure multiplies the near-wall
+streamfunction by A = 2(1/h1^2 + 1/(h1 h2)) ~ 1e4-1e5, so two FP64 solves that agree to rounding
+give wall vorticities that differ by A times that rounding. The step comparison is therefore
+split: psi and the interior rows of omega against the oracle, and the wall rows of omega against
+the closure applied by the oracle to the GPU's own psi."""
 import ctypes
 import math
 
@@ -16,17 +23,40 @@ pytestmark = pytest.mark.gpu
 L2PI = 2 * math.pi
 
 
-class OracleStepper:
-    """The oracle's restatement of one SSPRK(3,3) step (main.cu:482-538)."""
+class GpuPoisson:
+    """The HIP Poisson solver (poisson2d_* C-ABI) as a host-array function."""
 
-    def __init__(self, g, nu, F0, nforce, drag, psi_top, free_slip=0, mms=None):
+    def __init__(self, g):
+        self.g = g
+        self.p = nh.lib().poisson2d_create(g.Nx, g.Ny, g.Lx, None)
+        assert self.p, nh.last_error()
+        assert nh.lib().poisson2d_enable_mapped(self.p, g.dp("a_node"), g.dp("a_edge"), g.inv_deta, None) == 0
+        self.dw, self.dpsi = nh.DeviceBuffer(g.Nx * g.Ny), nh.DeviceBuffer(g.Nx * g.Ny)
+
+    def __call__(self, w, psi_top):
+        self.dw.upload(w)
+        nh.lib().poisson2d_solve(self.p, self.dw.ptr, self.dpsi.ptr, 0.0, 0.0, psi_top, None)
+        return self.dpsi.download(w.shape)
+
+    def close(self):
+        nh.lib().poisson2d_destroy(self.p)
+
+
+class OracleStepper:
+    """The oracle's restatement of one SSPRK(3,3) step (main.cu:482-538). With `solver` given,
+    the Poisson solves use it instead of the oracle's (composition checks)."""
+
+    def __init__(self, g, nu, F0, nforce, drag, psi_top, free_slip=0, mms=None, solver=None):
         self.g, self.nu, self.F0, self.nforce, self.drag = g, nu, F0, nforce, drag
-        self.psi_top, self.fs, self.mms = psi_top, free_slip, mms
+        self.psi_top, self.fs, self.mms, self.solver = psi_top, free_slip, mms, solver
         n = (g.Ny, g.Nx)
         self.J, self.lap, self.rhs = np.zeros(n), np.zeros(n), np.zeros(n)
 
     def solve(self, w):
-        psi = oracle.poisson_solve(w, self.g.Lx, self.g.m, 0.0, self.psi_top)
+        if self.solver is not None:
+            psi = self.solver(w, self.psi_top)
+        else:
+            psi = oracle.poisson_solve(w, self.g.Lx, self.g.m, 0.0, self.psi_top)
         oracle.lib().orc_thom_wall_vorticity_mapped(P(w), P(psi), self.g.Nx, self.g.Ny, P(self.g.m.dy_edge), 0.0,
                                                     self.psi_top, self.fs)
         return psi
@@ -77,6 +107,15 @@ def field(ch, which, g):
     return nh.download_ptr(nh.lib().nse2d_channel_field(ch, which), g.Nx * g.Ny, (g.Ny, g.Nx))
 
 
+def step_errors(g, w_gpu, psi_gpu, w_o, psi_o, psi_top, free_slip=0):
+    """(psi, interior omega, wall omega vs closure of the GPU psi, wall omega vs oracle)."""
+    w_cl = w_gpu.copy()
+    oracle.lib().orc_thom_wall_vorticity_mapped(P(w_cl), P(psi_gpu), g.Nx, g.Ny, P(g.m.dy_edge), 0.0, psi_top,
+                                                free_slip)
+    return (rel_err(psi_gpu, psi_o), rel_err(w_gpu[1:-1], w_o[1:-1]),
+            rel_err(w_gpu[[0, -1]], w_cl[[0, -1]]), rel_err(w_gpu[[0, -1]], w_o[[0, -1]]))
+
+
 def channel_ic(Nx, Ny, Lx=L2PI):
     args = ["--Nx", str(Nx), "--Ny", str(Ny), "--Lx", repr(Lx), "--init", "lamband", "--amp", "0.05",
             "--band_kmax", str(min(32, Nx // 4)), "--stretch", "tanh", "--beta", "2", "--Ub", "1", "--seed", "7"]
@@ -96,15 +135,45 @@ def test_one_step_matches_oracle(Nx, Ny, mms):
     nh.upload_ptr(nh.lib().nse2d_channel_field(ch, nh.FIELD_OMEGA), w0)
     nh.lib().nse2d_channel_init(ch)
     sync()
-    assert rel_err(field(ch, nh.FIELD_PSI, g), psi_o) <= 1e-13
-    assert rel_err(field(ch, nh.FIELD_OMEGA, g), w_o) <= 1e-13
+    e0 = step_errors(g, field(ch, nh.FIELD_OMEGA, g), field(ch, nh.FIELD_PSI, g), w_o, psi_o, 2.0)
+    assert e0[0] <= 1e-13 and e0[1] == 0.0 and e0[2] <= 1e-13
     t, dt = 0.3, 1e-3
     psi_o, w_o = o.step(psi_o, w_o, t, dt)
     nh.lib().nse2d_channel_step(ch, t, dt, nh.STEP_STORE_RHS)
     sync()
-    assert rel_err(field(ch, nh.FIELD_OMEGA, g), w_o) <= 1e-13
-    assert rel_err(field(ch, nh.FIELD_PSI, g), psi_o) <= 1e-13
-    assert rel_err(field(ch, nh.FIELD_RHS, g), o.rhs) <= 1e-13
+    e_psi, e_int, e_cl, e_wall = step_errors(g, field(ch, nh.FIELD_OMEGA, g), field(ch, nh.FIELD_PSI, g), w_o, psi_o, 2.0)
+    e_rhs = rel_err(field(ch, nh.FIELD_RHS, g)[1:-1], o.rhs[1:-1])
+    print(f"[step] {Nx}x{Ny} psi {e_psi:.2e} omega interior {e_int:.2e} rhs interior {e_rhs:.2e} "
+          f"wall vs closure {e_cl:.2e} wall vs oracle {e_wall:.2e}", flush=True)
+    assert e_psi <= 2e-13 and e_cl <= 1e-13
+    assert e_int <= 5e-12 and e_rhs <= 5e-10 and e_wall <= 1e-10  # closure conditioning, see module doc
+    nh.lib().nse2d_channel_destroy(ch)
+
+
+@pytest.mark.parametrize("Nx,Ny,mms", [(64, 33, None), (256, 129, None), (128, 65, (1.0, 0.1, 1.0, 2))])
+def test_step_composition_with_same_solver(Nx, Ny, mms):
+    """The fused step equals the oracle's unfused stage sequence (J, Laplacian, RHS, MMS, SSPRK
+    combinations, Thom) to 1e-13 on every row when both sides use the same (HIP) Poisson solver."""
+    g = Grid(Nx, Ny, 2.0)
+    nu, F0, nf, drag = 1e-3, 0.05, 12, 1e-3
+    w0 = channel_ic(Nx, Ny)
+    solver = GpuPoisson(g)
+    o = OracleStepper(g, nu, F0, nf, drag, 2.0, mms=mms, solver=solver)
+    w_o = w0.copy()
+    psi_o = o.solve(w_o)
+    ch = make_channel(g, nu, F0, nf, drag, 2.0, mms=mms)
+    nh.upload_ptr(nh.lib().nse2d_channel_field(ch, nh.FIELD_OMEGA), w0)
+    nh.lib().nse2d_channel_init(ch)
+    t, dt = 0.3, 1e-3
+    for _ in range(3):
+        psi_o, w_o = o.step(psi_o, w_o, t, dt)
+        nh.lib().nse2d_channel_step(ch, t, dt, nh.STEP_STORE_RHS)
+        t += dt
+    sync()
+    errs = [rel_err(field(ch, k, g), r) for k, r in ((nh.FIELD_OMEGA, w_o), (nh.FIELD_PSI, psi_o), (nh.FIELD_RHS, o.rhs))]
+    print(f"[composition] {Nx}x{Ny} 3 steps: omega {errs[0]:.2e} psi {errs[1]:.2e} rhs {errs[2]:.2e}", flush=True)
+    assert max(errs) <= 1e-13
+    solver.close()
     nh.lib().nse2d_channel_destroy(ch)
 
 
@@ -129,9 +198,10 @@ def test_thousand_steps_within_1e10():
         if k % 250 == 249:
             print(f"[1000 steps] step {k + 1}", flush=True)
     sync()
-    e_w = rel_err(field(ch, nh.FIELD_OMEGA, g), w_o)
-    e_p = rel_err(field(ch, nh.FIELD_PSI, g), psi_o)
-    print(f"[1000 steps] omega {e_w:.2e} psi {e_p:.2e}", flush=True)
+    w_g, psi_g = field(ch, nh.FIELD_OMEGA, g), field(ch, nh.FIELD_PSI, g)
+    e_w, e_p = rel_err(w_g, w_o), rel_err(psi_g, psi_o)
+    e_int = rel_err(w_g[1:-1], w_o[1:-1])
+    print(f"[1000 steps] omega {e_w:.2e} (interior rows {e_int:.2e}) psi {e_p:.2e}", flush=True)
     assert e_w <= 1e-10 and e_p <= 1e-10
     nh.lib().nse2d_channel_destroy(ch)
 
@@ -168,5 +238,5 @@ def test_device_dt_matches_host_rule():
         assert abs(dt_dev - dt) <= 1e-15 * dt, (k, dt_dev, dt)
     t_dev = nh.download_ptr(nh.lib().nse2d_channel_time_ptr(ch), 1)[0]
     assert abs(t_dev - t) <= 1e-14
-    assert rel_err(field(ch, nh.FIELD_OMEGA, g), w_o) <= 1e-12
+    assert rel_err(field(ch, nh.FIELD_OMEGA, g), w_o) <= 1e-10
     nh.lib().nse2d_channel_destroy(ch)
