@@ -104,7 +104,7 @@ def cpu_baseline(c, budget_s):
     L.orc_num_threads.restype = ctypes.c_int
     args = ic_args(c)
     t1 = L.orc_time_steps(len(args), cargv(args), 1, 1)  # includes one warm step
-    steps = int(max(1, min(20, budget_s / max(t1, 1e-3))))
+    steps = int(max(1, min(1000, budget_s / max(t1, 1e-3))))
     el = L.orc_time_steps(len(args), cargv(args), 1, steps)
     pts = c["Nx"] * c["Ny"] * steps
     return {"value": pts / el, "unit": UNIT, "cores": int(L.orc_num_threads()), "kind": "port",
@@ -119,6 +119,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first run 5 device-to-device copies of 256 MiB (a plain copy kernel of known size, "
+                         "for scaling HBM counter readings)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {a.gpus} is not supported (the channel "
@@ -126,6 +129,14 @@ def main():
         sys.exit(2)
     nh.require_gpu()
     L = nh.lib()
+    if a.copy_calibration:
+        n = 256 << 20
+        src, dst = nh.DeviceBuffer(n // 8, init=np.ones(n // 8)), nh.DeviceBuffer(n // 8)
+        for _ in range(5):
+            L.nse2d_memcpy_d2d(dst.ptr, src.ptr, n)
+        L.nse2d_device_sync()
+        src.free()
+        dst.free()
     c = CFG
     Nx, Ny = c["Nx"], c["Ny"]
     ch, keep = setup_channel(c)
@@ -156,10 +167,10 @@ def main():
     bytes_per_launch = ALG_BYTES[dom] * Nx * Ny
     achieved = bytes_per_launch / avg_s / 1e9
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_{dom}.json")
+    tpath = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.isfile(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("bytes_per_launch")
+            traffic = json.load(f).get("bytes_per_launch", {}).get(dom)
     out = {
         "metric": METRIC, "value": value, "unit": UNIT, "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": elapsed / a.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
