@@ -7,6 +7,8 @@
 #include "nse2d_poisson.hpp"
 #include "../../include/nse2d_hip.h"
 
+#include <cstdio>
+#include <cstdlib>
 #include <string>
 #include <utility>
 #include <vector>
@@ -44,11 +46,13 @@ constexpr int SX = 64, SY = 4, RY = 8;
 //   stage 1: out = ws + dt R;  stage 2: out = 3/4 wn + 1/4 (ws + dt R);
 //   stage 3: out = 1/3 wn + 2/3 (ws + dt R)   (out may alias wn: each point reads then writes
 //   its own element of wn only)
-template <int STAGE>
+template <int STAGE, bool MMS>
 __global__ void __launch_bounds__(256) stage_kernel(StageArgs a) {
     const int Nx = a.Nx, Ny = a.Ny;
-    const int ix = blockIdx.x * SX + threadIdx.x;
-    const int jlo = (blockIdx.y * SY + threadIdx.y) * RY;
+    const int ntx = (Nx + SX - 1) / SX;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);  // row-major tiles, contiguous per XCD
+    const int ix = (tile % ntx) * SX + threadIdx.x;
+    const int jlo = ((tile / ntx) * SY + threadIdx.y) * RY;
     if (ix >= Nx || jlo >= Ny) return;
     const int jhi = min(jlo + RY, Ny);
     const int xm = wrap_x(ix - 1, Nx), xp = wrap_x(ix + 1, Nx);
@@ -84,7 +88,7 @@ __global__ void __launch_bounds__(256) stage_kernel(StageArgs a) {
         }
         const double wc = W.c;
         double rhs = -J + a.nu * lap + a.f_row[j] - a.lin_drag * wc;
-        if (a.mms) rhs += mms_f_omega(a.mp, ix, a.y_node[j]);
+        if (MMS) rhs += mms_f_omega(a.mp, ix, a.y_node[j]);
         const size_t c = idx2(j, ix, Nx);
         if (a.rhs_out) a.rhs_out[c] = rhs;
         double o;
@@ -93,6 +97,87 @@ __global__ void __launch_bounds__(256) stage_kernel(StageArgs a) {
         else o = (1.0 / 3.0) * a.wn[c] + (2.0 / 3.0) * (wc + dt * rhs);
         a.out[c] = o;
         // slide the window down one row
+        P.mm = P.xm; P.ym = P.c; P.pm = P.xp; P.xm = P.mp; P.c = P.yp; P.xp = P.pp;
+        W.mm = W.xm; W.ym = W.c; W.pm = W.xp; W.xm = W.mp; W.c = W.yp; W.xp = W.pp;
+    }
+}
+
+// The same stage for Nx a multiple of 64: one wavefront per 64-column strip of RY rows, SY strips
+// stacked in y per workgroup. Each row of psi and omega_s costs one coalesced load per array plus
+// a two-lane load of the strip's halo columns (x0-1 for lane 0, x0+64 for lane 63); the x-1 and
+// x+1 neighbours come from DPP lane shifts. Row j+2 (and omega^n of row j+1) is fetched while
+// row j is computed, so every load has a full row of arithmetic to arrive.
+template <int STAGE, bool MMS, int RY, int SYW>
+__global__ void __launch_bounds__(64 * SYW) stage_kernel_wave(StageArgs a) {
+    const int Nx = a.Nx, Ny = a.Ny;
+    const int lane = threadIdx.x;
+    const int ntx = Nx >> 6;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int x0 = (tile % ntx) << 6, ix = x0 + lane;
+    const int jlo = ((tile / ntx) * SYW + threadIdx.y) * RY;
+    if (jlo >= Ny) return;  // uniform over the wavefront
+    const int jhi = min(jlo + RY, Ny);
+    const bool edge = (lane == 0) || (lane == 63);
+    const int hx = (lane == 0) ? (x0 == 0 ? Nx - 1 : x0 - 1) : (x0 + 64 == Nx ? 0 : x0 + 64);
+    const double inv4dxde = 0.25 * a.inv_dx * a.inv_deta;
+    const double dt = a.dt_ptr ? *a.dt_ptr : a.dt_val;
+    struct Raw { double pc, ph, wc, wh; };
+    auto load = [&](int j, Raw& r) {
+        const double* __restrict__ pr = a.psi + (size_t)j * Nx;
+        const double* __restrict__ wr = a.ws + (size_t)j * Nx;
+        r.pc = pr[ix];
+        r.wc = wr[ix];
+        r.ph = 0.0;
+        r.wh = 0.0;
+        if (edge) { r.ph = pr[hx]; r.wh = wr[hx]; }
+    };
+    // (x-1, x, x+1) of a loaded row
+    auto expand = [&](const Raw& r, double& pm, double& pc, double& pp, double& wm, double& wc, double& wp) {
+        pm = wave_shr1(r.pc, r.ph); pc = r.pc; pp = wave_shl1(r.pc, r.ph);
+        wm = wave_shr1(r.wc, r.wh); wc = r.wc; wp = wave_shl1(r.wc, r.wh);
+    };
+    Stencil9 P, W;
+    Raw nxt;
+    if (jlo > 0) {
+        Raw r;
+        load(jlo - 1, r);
+        expand(r, P.mm, P.ym, P.pm, W.mm, W.ym, W.pm);
+    }
+    {
+        Raw r;
+        load(jlo, r);
+        expand(r, P.xm, P.c, P.xp, W.xm, W.c, W.xp);
+    }
+    if (jlo + 1 < Ny) load(jlo + 1, nxt);
+    double wn_next = (STAGE > 1) ? a.wn[idx2(jlo, ix, Nx)] : 0.0;
+#pragma unroll
+    for (int q = 0; q < RY; ++q) {
+        const int j = jlo + q;
+        if (j >= jhi) break;
+        const Raw cur = nxt;  // row j+1
+        const double wn_c = wn_next;
+        if (q + 1 < RY && j + 2 < Ny) load(j + 2, nxt);
+        if (STAGE > 1 && q + 1 < RY && j + 1 < jhi) wn_next = a.wn[idx2(j + 1, ix, Nx)];
+        if (j + 1 < Ny) expand(cur, P.mp, P.yp, P.pp, W.mp, W.yp, W.pp);
+        double J = 0.0, lap = 0.0;
+        if (j > 0 && j < Ny - 1) {
+            const double an = a.a_node[j];
+            double J1, J2, J3;
+            arakawa_terms(P, W, an, inv4dxde, J1, J2, J3);
+            J = -((J1 + J2 + J3) * (1.0 / 3.0));
+            lap = mapped_laplacian(W.xm, W.c, W.xp, W.ym, W.yp, an, a.a_edge[j], a.a_edge[j - 1], a.inv_dx2,
+                                   a.inv_deta2);
+        }
+        const double wc = W.c;
+        double rhs = -J + a.nu * lap + a.f_row[j] - a.lin_drag * wc;
+        if (MMS) rhs += mms_f_omega(a.mp, ix, a.y_node[j]);
+        const size_t c = idx2(j, ix, Nx);
+        if (a.rhs_out) a.rhs_out[c] = rhs;
+        double o;
+        if (STAGE == 1) o = wc + dt * rhs;
+        else if (STAGE == 2) o = 0.75 * wn_c + 0.25 * (wc + dt * rhs);
+        else o = (1.0 / 3.0) * wn_c + (2.0 / 3.0) * (wc + dt * rhs);
+        a.out[c] = o;
         P.mm = P.xm; P.ym = P.c; P.pm = P.xp; P.xm = P.mp; P.c = P.yp; P.xp = P.pp;
         W.mm = W.xm; W.ym = W.c; W.pm = W.xp; W.xm = W.mp; W.c = W.yp; W.xp = W.pp;
     }
@@ -156,6 +241,7 @@ struct ChannelImpl {
     PoissonImpl* P = nullptr;
     double inv_dx = 0, inv_deta = 0, inv_dx2 = 0, inv_deta2 = 0, dx = 0, deta = 0;
     // optional timing of kernel classes with events on the launch stream
+    int stage_tile = 804;  // wavefront stage kernel shape RY*100 + SYW (0: generic kernel)
     int prof_mask = 0;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
@@ -209,11 +295,51 @@ StageArgs stage_args(ChannelImpl* c, const double* ws, const double* wn, double*
     return a;
 }
 
+// Tile shape of the wavefront stage kernel, RY rows per wavefront x SYW wavefronts per
+// workgroup; NSE2D_STAGE_TILE=RYxSYW selects another instantiated shape (used to tune on the
+// device), 0 selects the generic kernel.
+int stage_tile_code() {
+    int ry = 8, sy = 4;
+    if (const char* e = std::getenv("NSE2D_STAGE_TILE")) {
+        if (e[0] == '0') return 0;
+        if (std::sscanf(e, "%dx%d", &ry, &sy) != 2) { ry = 8; sy = 4; }
+    }
+    const int c = ry * 100 + sy;
+    switch (c) {
+        case 804: case 1604: case 1602: case 1601: case 3201: case 3202: case 808: return c;
+        default: return 804;
+    }
+}
+
+// The manufactured-solution forcing (sin/cos per point) is a separate instantiation so that the
+// production kernels carry none of its registers.
+template <int STAGE, bool MMS>
+void launch_stage_t(ChannelImpl* c, const StageArgs& a) {
+    const int code = (c->d.Nx % 64 == 0) ? c->stage_tile : 0;
+#define NSE2D_STAGE_WAVE(RYV, SYV)                                                                 \
+    stage_kernel_wave<STAGE, MMS, RYV, SYV><<<(c->d.Nx / 64) * ceil_div(c->d.Ny, RYV * SYV),       \
+                                              dim3(64, SYV), 0, c->s>>>(a)
+    switch (code) {
+        case 804: NSE2D_STAGE_WAVE(8, 4); break;
+        case 808: NSE2D_STAGE_WAVE(8, 8); break;
+        case 1604: NSE2D_STAGE_WAVE(16, 4); break;
+        case 1602: NSE2D_STAGE_WAVE(16, 2); break;
+        case 1601: NSE2D_STAGE_WAVE(16, 1); break;
+        case 3201: NSE2D_STAGE_WAVE(32, 1); break;
+        case 3202: NSE2D_STAGE_WAVE(32, 2); break;
+        default: {
+            const unsigned ntiles = ceil_div(c->d.Nx, SX) * ceil_div(c->d.Ny, SY * RY);
+            stage_kernel<STAGE, MMS><<<ntiles, dim3(SX, SY), 0, c->s>>>(a);
+        }
+    }
+#undef NSE2D_STAGE_WAVE
+}
+
 template <int STAGE>
 void launch_stage(ChannelImpl* c, const StageArgs& a) {
     ProfScope ps(c, NSE2D_PROF_STAGE);
-    const dim3 grid(ceil_div(c->d.Nx, SX), ceil_div(c->d.Ny, SY * RY));
-    stage_kernel<STAGE><<<grid, dim3(SX, SY), 0, c->s>>>(a);
+    if (a.mms) launch_stage_t<STAGE, true>(c, a);
+    else launch_stage_t<STAGE, false>(c, a);
     NSE2D_LAUNCH_CHECK();
 }
 
@@ -269,6 +395,7 @@ nse2d_channel_t nse2d_channel_create(const nse2d_channel_desc* desc, hipStream_t
     c->d = d;
     c->s = s;
     c->P = P;
+    c->stage_tile = stage_tile_code();
     c->dx = d.Lx / (double)d.Nx;
     c->inv_dx = 1.0 / c->dx;
     c->inv_dx2 = c->inv_dx * c->inv_dx;

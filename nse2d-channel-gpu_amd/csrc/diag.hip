@@ -255,7 +255,7 @@ __global__ void __launch_bounds__(RB) row_max_uv_kernel(const double* __restrict
                                                         const double* __restrict__ a_node,
                                                         double* __restrict__ partial) {
     __shared__ double sm[3 * 16];
-    const int j = blockIdx.x;
+    const int j = xcd_tile(blockIdx.x, gridDim.x);  // neighbouring rows share an L2
     double m[3] = {0, 0, 0};
     if (j > 0 && j < Ny - 1) {
         const double a = a_node[j];

@@ -1,0 +1,227 @@
+// Row FFTs of the Poisson solve for M = Nx/2 in [256, 8192]: one workgroup per row, T = M/R
+// threads, each holding R complex values in registers (R = 4, 8 or 16).
+//
+// Stockham autosort passes p = 0..NPASS-1 of radix Rp (R, the last pass 2^(log2 M mod log2 R)
+// when R does not divide the exponent), sub-transform size Ns = prod of the earlier radices:
+//   item j (0 <= j < M/Rp), k = j mod Ns:  v_q = x[j + q M/Rp] * W_{Rp Ns}^{q k}  (q < Rp)
+//   y[(j - k) Rp + k + q Ns] = DFT_Rp(v)_q
+// Data move through LDS between passes; the forward transform's first pass reads the row from
+// HBM straight into registers and the inverse's last pass writes the row to HBM from registers,
+// so a row costs one HBM read and one HBM write and NPASS-1 LDS exchanges (two for M = 2048).
+// LDS index p is stored at p + p/16: the stride-Rp stores of the first pass and the (Ns >= 16)
+// blocked stores of the later passes then fall on distinct banks.
+//
+// Real rows: the Nx real values are the M complex values z_n = (x_2n, x_2n+1); the half spectrum
+// X[0..M] follows from Z by the usual split (forward) and is rebuilt before the inverse:
+//   X_k = (Z_k + conj Z_{M-k})/2 - i/2 W^k (Z_k - conj Z_{M-k}),  W = exp(-2 pi i / Nx)
+//   Z_k = (X_k + conj X_{M-k}) + i conj(W^k) (X_k - conj X_{M-k})   (twice the true Z)
+// with X_0 and X_M (both real) packed into slot 0 as (X_0, X_M).
+#pragma once
+#include "nse2d_common.hpp"
+
+#include <utility>
+
+namespace nse2d {
+namespace fft {
+
+struct cplx { double x, y; };
+__device__ inline cplx cadd(cplx a, cplx b) { return {a.x + b.x, a.y + b.y}; }
+__device__ inline cplx csub(cplx a, cplx b) { return {a.x - b.x, a.y - b.y}; }
+__device__ inline cplx cmul(cplx a, cplx b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+__device__ inline cplx cmulc(cplx a, cplx b) { return {a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y}; }  // a conj(b)
+__device__ inline cplx cconj(cplx a) { return {a.x, -a.y}; }
+__device__ inline cplx cscale(cplx a, double s) { return {a.x * s, a.y * s}; }
+
+constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n >> 1); }
+
+template <int M, int R>
+struct Plan {
+    static_assert((M & (M - 1)) == 0 && (R == 4 || R == 8 || R == 16) && M >= R * 64, "unsupported row FFT plan");
+    static constexpr int T = M / R;
+    static constexpr int LOGM = ilog2c(M), LOGR = ilog2c(R);
+    static constexpr int NFULL = LOGM / LOGR, REM = LOGM % LOGR;
+    static constexpr int NPASS = NFULL + (REM ? 1 : 0);
+    static constexpr int LDS = M + (M >> 4);
+    static constexpr int radix(int p) { return p < NFULL ? R : (1 << REM); }
+    static constexpr int ns(int p) {
+        int n = 1;
+        for (int i = 0; i < p; ++i) n *= radix(i);
+        return n;
+    }
+    // offset of pass p's twiddle block in the plan table: entries [k][q-1], k < Ns, 1 <= q < Rp
+    static constexpr int twoff(int p) {
+        int o = 0;
+        for (int i = 1; i < p; ++i) o += ns(i) * (radix(i) - 1);
+        return o;
+    }
+    static constexpr int TWSIZE = twoff(NPASS);
+};
+
+__device__ inline int lds_pad(int p) { return p + (p >> 4); }
+
+// multiply by -i (forward) or +i (inverse)
+template <bool INV>
+__device__ inline cplx mul_mi(cplx a) { return INV ? cplx{-a.y, a.x} : cplx{a.y, -a.x}; }
+
+// a * exp(-+2 pi i e / N) for N in {8, 16}; e is a constant after unrolling, so the table
+// reads fold into immediates and the exact cases (multiples of pi/2) skip the multiply.
+template <int N, bool INV>
+__device__ inline cplx twiddle_const(cplx a, int e) {
+    // cos and sin of m pi / 8, m = 0..15
+    constexpr double C1 = 0.92387953251128675613, S1 = 0.38268343236508977173, R2 = 0.70710678118654752440;
+    constexpr double cs[16] = {1, C1, R2, S1, 0, -S1, -R2, -C1, -1, -C1, -R2, -S1, 0, S1, R2, C1};
+    constexpr double sn[16] = {0, S1, R2, C1, 1, C1, R2, S1, 0, -S1, -R2, -C1, -1, -C1, -R2, -S1};
+    const int m = (e * (16 / N)) & 15;
+    if (m == 0) return a;
+    if (m == 8) return {-a.x, -a.y};
+    if (m == 4) return mul_mi<INV>(a);
+    if (m == 12) return mul_mi<!INV>(a);
+    const double c = cs[m], s = sn[m];
+    // forward: a (c - i s); inverse: a (c + i s)
+    return INV ? cplx{a.x * c - a.y * s, a.y * c + a.x * s} : cplx{a.x * c + a.y * s, a.y * c - a.x * s};
+}
+
+// In-register DFT of N = 1, 2, 4, 8 or 16 values x[0], x[S], .., x[(N-1)S], natural order in and
+// out. N = N1 * 4: DFT-4 over n2 of x[n1 + N1 n2], twiddle W_N^{n1 k2}, DFT-N1 over n1, and the
+// result X[4 k1 + k2] (found at n1 = k1, slot k2) is moved to natural order.
+template <int N, bool INV, int S = 1>
+__device__ inline void dft(cplx* x) {
+    if constexpr (N == 2) {
+        const cplx a = x[0], b = x[S];
+        x[0] = cadd(a, b);
+        x[S] = csub(a, b);
+    } else if constexpr (N == 4) {
+        const cplx a0 = x[0], a1 = x[S], a2 = x[2 * S], a3 = x[3 * S];
+        const cplx t0 = cadd(a0, a2), t1 = csub(a0, a2), t2 = cadd(a1, a3), t3 = mul_mi<INV>(csub(a1, a3));
+        x[0] = cadd(t0, t2);
+        x[S] = cadd(t1, t3);
+        x[2 * S] = csub(t0, t2);
+        x[3 * S] = csub(t1, t3);
+    } else if constexpr (N == 8 || N == 16) {
+        constexpr int N1 = N / 4;
+#pragma unroll
+        for (int n1 = 0; n1 < N1; ++n1) dft<4, INV, N1 * S>(x + n1 * S);
+#pragma unroll
+        for (int n1 = 1; n1 < N1; ++n1)
+#pragma unroll
+            for (int k2 = 1; k2 < 4; ++k2) x[(n1 + N1 * k2) * S] = twiddle_const<N, INV>(x[(n1 + N1 * k2) * S], n1 * k2);
+#pragma unroll
+        for (int k2 = 0; k2 < 4; ++k2) dft<N1, INV, S>(x + N1 * k2 * S);
+        cplx t[N];
+#pragma unroll
+        for (int k1 = 0; k1 < N1; ++k1)
+#pragma unroll
+            for (int k2 = 0; k2 < 4; ++k2) t[4 * k1 + k2] = x[(k1 + N1 * k2) * S];
+#pragma unroll
+        for (int n = 0; n < N; ++n) x[n * S] = t[n];
+    }
+}
+
+// Pass p >= 1 (or p = 0 of the inverse) from LDS; writes LDS, or the output row for the
+// inverse's last pass (OUT != nullptr, scaled).
+template <int M, int R, bool INV, int p>
+__device__ inline void lds_pass(cplx (&v)[R], cplx* lds, const cplx* __restrict__ twp, cplx* __restrict__ out,
+                                double scale) {
+    using P = Plan<M, R>;
+    constexpr int T = P::T, Rp = P::radix(p), Ns = P::ns(p), NI = R / Rp, MR = M / Rp;
+    constexpr bool LAST = (p == P::NPASS - 1);
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int i = 0; i < NI; ++i)
+#pragma unroll
+        for (int q = 0; q < Rp; ++q) v[i * Rp + q] = lds[lds_pad(tid + i * T + q * MR)];
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < NI; ++i) {
+        const int j = tid + i * T, k = j & (Ns - 1);
+        if constexpr (Ns > 1) {
+            const cplx* w = twp + P::twoff(p) + k * (Rp - 1);
+#pragma unroll
+            for (int q = 1; q < Rp; ++q) v[i * Rp + q] = INV ? cmulc(v[i * Rp + q], w[q - 1]) : cmul(v[i * Rp + q], w[q - 1]);
+        }
+        dft<Rp, INV>(v + i * Rp);
+        const int base = (j - k) * Rp + k;
+        if (LAST && INV) {
+            // last pass: j < Ns, so base = j and the stores are contiguous across threads
+#pragma unroll
+            for (int q = 0; q < Rp; ++q) out[base + q * Ns] = cscale(v[i * Rp + q], scale);
+        } else {
+#pragma unroll
+            for (int q = 0; q < Rp; ++q) lds[lds_pad(base + q * Ns)] = v[i * Rp + q];
+        }
+    }
+    if (!(LAST && INV)) __syncthreads();
+}
+
+template <int M, int R, bool INV, int... Ps>
+__device__ inline void lds_passes(cplx (&v)[R], cplx* lds, const cplx* twp, cplx* out, double scale,
+                                  std::integer_sequence<int, Ps...>) {
+    (lds_pass<M, R, INV, Ps + (INV ? 0 : 1)>(v, lds, twp, out, scale), ...);
+}
+
+// Forward: real rows of omega ([rows][2M]) -> packed half spectra ([rows][M]).
+template <int M, int R>
+__global__ void __launch_bounds__(M / R) rows_forward(const double* __restrict__ omega, cplx* __restrict__ hat,
+                                                      const cplx* __restrict__ twp, const cplx* __restrict__ tw) {
+    using P = Plan<M, R>;
+    constexpr int T = P::T;
+    __shared__ cplx lds[P::LDS];
+    const int tid = threadIdx.x;
+    const size_t row = blockIdx.x;
+    const cplx* src = reinterpret_cast<const cplx*>(omega + row * 2 * M);
+    cplx v[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) v[q] = src[tid + q * T];
+    dft<R, false>(v);  // pass 0: Ns = 1, no twiddles
+#pragma unroll
+    for (int q = 0; q < R; ++q) lds[lds_pad(tid * R + q)] = v[q];
+    __syncthreads();
+    lds_passes<M, R, false>(v, lds, twp, nullptr, 1.0, std::make_integer_sequence<int, P::NPASS - 1>{});
+    cplx* dst = hat + row * M;
+    for (int k = tid; k <= M / 2; k += T) {
+        const cplx Zk = lds[lds_pad(k)], Zm = lds[lds_pad((M - k) & (M - 1))];
+        if (k == 0) {
+            dst[0] = {Zk.x + Zk.y, Zk.x - Zk.y};  // (X[0], X[M])
+            continue;
+        }
+        const cplx E = cscale(cadd(Zk, cconj(Zm)), 0.5);
+        const cplx D = csub(Zk, cconj(Zm));
+        const cplx O = {0.5 * D.y, -0.5 * D.x};  // -i/2 * D
+        const cplx WO = cmul(tw[k], O);
+        dst[k] = cadd(E, WO);
+        if (k != M - k) dst[M - k] = cconj(csub(E, WO));
+    }
+}
+
+// Inverse: packed half spectra -> real rows, scaled by `scale` (1/Nx).
+template <int M, int R>
+__global__ void __launch_bounds__(M / R) rows_inverse(const cplx* __restrict__ hat, double* __restrict__ psi,
+                                                      const cplx* __restrict__ twp, const cplx* __restrict__ tw,
+                                                      double scale) {
+    using P = Plan<M, R>;
+    constexpr int T = P::T;
+    __shared__ cplx lds[P::LDS];
+    const int tid = threadIdx.x;
+    const size_t row = blockIdx.x;
+    const cplx* src = hat + row * M;
+    for (int k = tid; k <= M / 2; k += T) {
+        if (k == 0) {
+            const cplx p = src[0];  // (X[0], X[M])
+            lds[0] = {p.x + p.y, p.x - p.y};
+            continue;
+        }
+        const cplx Xk = src[k], Xm = src[M - k];
+        const cplx A = cadd(Xk, cconj(Xm)), B = csub(Xk, cconj(Xm));
+        const cplx WB = cmulc(B, tw[k]);          // conj(W^k) B
+        const cplx iWB = {-WB.y, WB.x};
+        lds[lds_pad(k)] = cadd(A, iWB);
+        if (k != M - k) lds[lds_pad(M - k)] = cconj(csub(A, iWB));
+    }
+    __syncthreads();
+    cplx v[R];
+    cplx* dst = reinterpret_cast<cplx*>(psi + row * 2 * M);
+    lds_passes<M, R, true>(v, lds, twp, dst, scale, std::make_integer_sequence<int, P::NPASS>{});
+}
+
+}  // namespace fft
+}  // namespace nse2d

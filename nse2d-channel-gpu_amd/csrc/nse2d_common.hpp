@@ -30,6 +30,17 @@ __host__ __device__ inline int wrap_x(int i, int Nx) { return i < 0 ? i + Nx : (
 
 inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
+// MI355X dispatches the workgroups of a 1-D grid round-robin over its 8 XCDs (workgroup b runs on
+// XCD b % 8), and each XCD has its own L2. xcd_tile(b, nb) renumbers the nb workgroups so that
+// each XCD receives one contiguous range of logical tiles, in order: tiles that share halo rows
+// or cache lines then meet in the same L2 instead of being fetched from HBM once per XCD.
+// The map is a bijection of [0, nb) for any nb.
+constexpr int kXcds = 8;
+__device__ inline int xcd_tile(int b, int nb) {
+    const int x = b % kXcds, q = nb / kXcds, r = nb % kXcds;
+    return x * q + min(x, r) + b / kXcds;
+}
+
 // Sum over the 64 lanes of a wavefront; every lane gets the total.
 __device__ inline double wave_sum(double v) {
 #pragma unroll
@@ -41,6 +52,24 @@ __device__ inline double wave_max(double v) {
     for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
     return v;
 }
+
+// Whole-wavefront lane shifts by one (GFX9 DPP wave_shr:1 / wave_shl:1, all 64 lanes active):
+// wave_shr1 gives lane i the value of lane i-1 and lane 0 its own `edge`; wave_shl1 gives lane i
+// the value of lane i+1 and lane 63 its own `edge`.
+__device__ inline double dpp_shift_(double v, double edge, int ctrl_is_shr) {
+    const unsigned long long vb = __double_as_longlong(v), eb = __double_as_longlong(edge);
+    int lo, hi;
+    if (ctrl_is_shr) {
+        lo = __builtin_amdgcn_update_dpp((int)(unsigned)eb, (int)(unsigned)vb, 0x138, 0xf, 0xf, false);
+        hi = __builtin_amdgcn_update_dpp((int)(unsigned)(eb >> 32), (int)(unsigned)(vb >> 32), 0x138, 0xf, 0xf, false);
+    } else {
+        lo = __builtin_amdgcn_update_dpp((int)(unsigned)eb, (int)(unsigned)vb, 0x130, 0xf, 0xf, false);
+        hi = __builtin_amdgcn_update_dpp((int)(unsigned)(eb >> 32), (int)(unsigned)(vb >> 32), 0x130, 0xf, 0xf, false);
+    }
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ inline double wave_shr1(double v, double edge) { return dpp_shift_(v, edge, 1); }
+__device__ inline double wave_shl1(double v, double edge) { return dpp_shift_(v, edge, 0); }
 
 // Block-wide sum of NACC accumulators; result valid in thread 0. blockDim.x must be a multiple
 // of 64 and at most 1024. `sm` needs NACC*16 doubles.

@@ -2,6 +2,9 @@
 // mapped second-order y operator, psi = psi_bot / psi_top on the walls.
 // Replaces poisson.cu + poisson.cuh (cuFFT Z2Z + one-thread-per-kx Thomas).
 //
+// Row FFTs: fft_rows.hpp (register radix-16/8/4 Stockham passes) for Nx/2 in [256, 8192]; the
+// LDS radix-4 kernels below for the smaller rows.
+//
 // Spectral layout: for each row j the Nx real values are transformed as one complex FFT of
 // M = Nx/2 points (even samples in the real part, odd samples in the imaginary part), then
 // split into the half spectrum X[0..M]. X[0] and X[M] are real for real data, so they share one
@@ -18,6 +21,7 @@
 // at the chunk boundaries (precomputed once per plan).
 #include "nse2d_common.hpp"
 #include "nse2d_poisson.hpp"
+#include "fft_rows.hpp"
 #include "../../include/nse2d_hip.h"
 
 #include <cmath>
@@ -31,12 +35,12 @@ void set_last_error(const std::string& msg);
 
 namespace {
 
-struct cplx { double x, y; };
-__device__ inline cplx cadd(cplx a, cplx b) { return {a.x + b.x, a.y + b.y}; }
-__device__ inline cplx csub(cplx a, cplx b) { return {a.x - b.x, a.y - b.y}; }
-__device__ inline cplx cmul(cplx a, cplx b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
-__device__ inline cplx cconj(cplx a) { return {a.x, -a.y}; }
-__device__ inline cplx cscale(cplx a, double s) { return {a.x * s, a.y * s}; }
+using fft::cplx;
+using fft::cadd;
+using fft::csub;
+using fft::cmul;
+using fft::cconj;
+using fft::cscale;
 
 // In-LDS Stockham FFT of M points (M a power of two), radix 4 with one leading radix-2 pass
 // when log2(M) is odd. Each pass: every thread loads its R inputs, barrier, then stores its R
@@ -363,7 +367,9 @@ struct PoissonImpl {
     double Lx = 0, inv_deta2 = 0;
     bool mapped = false, profile = false;
     float acc_fft_ms = 0.f, acc_tridiag_ms = 0.f;
+    int fft_radix = 0;         // register row FFT radix (0: the LDS radix-4 kernels)
     cplx* d_tw = nullptr;      // [Nx] exp(-2 pi i q / Nx)
+    cplx* d_twp = nullptr;     // per-pass twiddles of the register row FFT
     double* d_kx2 = nullptr;   // [M+1]
     double* d_cpb = nullptr;   // [NCH][M+1] elimination coefficient entering each chunk
     double* d_gny = nullptr;   // [Ny-2] g of the Nyquist system (slot 0, imaginary part)
@@ -374,6 +380,59 @@ struct PoissonImpl {
 };
 
 namespace {
+// Register row FFT plans (M = Nx/2, radix R): instantiated pairs and the default radix per M.
+// NSE2D_FFT_RADIX selects another instantiated radix (used to tune on the device).
+bool fft_instantiated(int M, int R) {
+    switch (M) {
+        case 256: return R == 4;
+        case 512: return R == 4 || R == 8;
+        case 1024: return R == 8 || R == 16;
+        case 2048: return R == 8 || R == 16;
+        case 4096: case 8192: return R == 16;
+        default: return false;
+    }
+}
+int choose_fft_radix(int M) {
+    if (const char* e = std::getenv("NSE2D_FFT_RADIX")) {
+        const int r = std::atoi(e);
+        if (r == 0 || fft_instantiated(M, r)) return r;
+    }
+    const int def = M <= 256 ? 4 : (M <= 2048 ? 8 : 16);
+    return fft_instantiated(M, def) ? def : 0;
+}
+// Twiddles of passes p >= 1 of the register FFT: [k][q-1] = exp(-2 pi i q k / (Rp Ns)).
+std::vector<cplx> fft_pass_twiddles(int M, int R) {
+    int logm = 0, logr = 0;
+    while ((1 << logm) < M) ++logm;
+    while ((1 << logr) < R) ++logr;
+    const int nfull = logm / logr, rem = logm % logr, npass = nfull + (rem ? 1 : 0);
+    std::vector<cplx> t;
+    int ns = R;
+    for (int p = 1; p < npass; ++p) {
+        const int rp = p < nfull ? R : (1 << rem);
+        for (int k = 0; k < ns; ++k)
+            for (int q = 1; q < rp; ++q) {
+                const long double ang = -2.0L * 3.14159265358979323846264338327950288L * (long double)(q * k) /
+                                        (long double)(rp * ns);
+                t.push_back({(double)std::cos(ang), (double)std::sin(ang)});
+            }
+        ns *= rp;
+    }
+    return t;
+}
+template <int M, int R>
+void launch_rows_forward(const double* omega, cplx* hat, int rows, const cplx* twp, const cplx* tw, hipStream_t s) {
+    static_assert(fft::Plan<M, R>::T >= 64, "row FFT needs at least one wavefront");
+    fft::rows_forward<M, R><<<rows, M / R, 0, s>>>(omega, hat, twp, tw);
+}
+template <int M, int R>
+void launch_rows_inverse(const cplx* hat, double* psi, int rows, const cplx* twp, const cplx* tw, double scale,
+                         hipStream_t s) {
+    fft::rows_inverse<M, R><<<rows, M / R, 0, s>>>(hat, psi, twp, tw, scale);
+}
+#define NSE2D_FFT_PLANS(X) X(256, 4) X(512, 4) X(512, 8) X(1024, 8) X(1024, 16) X(2048, 8) X(2048, 16) \
+    X(4096, 16) X(8192, 16)
+
 constexpr int kThomasThreads = 512;
 // Chunk geometry of the y solve: KC columns per workgroup, NCH = 512/KC chunks of RC rows
 // (RC a power of two, NCH * RC >= Ny-2). Instantiated: KC = 8 with RC <= 32, KC = 4 with
@@ -452,6 +511,13 @@ PoissonImpl* poisson_create(int Nx, int Ny, double Lx, hipStream_t s) {
         const double kphys = (2.0 * kPi * (double)k) / Lx;
         kx2[k] = kphys * kphys;
     }
+    p->fft_radix = choose_fft_radix(p->M);
+    if (p->fft_radix) {
+        const std::vector<cplx> twp = fft_pass_twiddles(p->M, p->fft_radix);
+        NSE2D_HIP_CHECK(hipMalloc(&p->d_twp, sizeof(cplx) * std::max<size_t>(1, twp.size())));
+        if (!twp.empty())
+            NSE2D_HIP_CHECK(hipMemcpyAsync(p->d_twp, twp.data(), sizeof(cplx) * twp.size(), hipMemcpyHostToDevice, s));
+    }
     NSE2D_HIP_CHECK(hipMalloc(&p->d_tw, sizeof(cplx) * Nx));
     NSE2D_HIP_CHECK(hipMalloc(&p->d_kx2, sizeof(double) * (p->M + 1)));
     NSE2D_HIP_CHECK(hipMalloc(&p->d_cpb, sizeof(double) * (size_t)p->nch * (p->M + 1)));
@@ -485,6 +551,17 @@ int poisson_enable_mapped(PoissonImpl* p, const double* a_node, const double* a_
 }
 
 void poisson_forward(PoissonImpl* p, const double* omega, hipStream_t s) {
+    if (p->fft_radix) {
+        switch (p->M * 100 + p->fft_radix) {
+#define NSE2D_FWD_CASE(M, R) \
+    case M * 100 + R: launch_rows_forward<M, R>(omega, p->d_hat, p->Ny, p->d_twp, p->d_tw, s); break;
+            NSE2D_FFT_PLANS(NSE2D_FWD_CASE)
+#undef NSE2D_FWD_CASE
+            default: std::abort();
+        }
+        NSE2D_LAUNCH_CHECK();
+        return;
+    }
     if (p->it4 == 1)
         fft_rows_forward_kernel<1><<<p->Ny, p->threads, p->lds_bytes, s>>>(omega, p->d_hat, p->Nx, p->log2M, p->d_tw);
     else
@@ -516,6 +593,17 @@ void poisson_tridiag(PoissonImpl* p, double psi_bot, double psi_top, hipStream_t
 }
 void poisson_inverse(PoissonImpl* p, double* psi, hipStream_t s) {
     const double scale = (double)(1.0 / (double)p->Nx);
+    if (p->fft_radix) {
+        switch (p->M * 100 + p->fft_radix) {
+#define NSE2D_INV_CASE(M, R) \
+    case M * 100 + R: launch_rows_inverse<M, R>(p->d_hat, psi, p->Ny, p->d_twp, p->d_tw, scale, s); break;
+            NSE2D_FFT_PLANS(NSE2D_INV_CASE)
+#undef NSE2D_INV_CASE
+            default: std::abort();
+        }
+        NSE2D_LAUNCH_CHECK();
+        return;
+    }
     if (p->it4 == 1)
         fft_rows_inverse_kernel<1><<<p->Ny, p->threads, p->lds_bytes, s>>>(p->d_hat, psi, p->Nx, p->log2M, p->d_tw, scale);
     else
@@ -556,7 +644,7 @@ void poisson_solve(PoissonImpl* p, const double* omega, double* psi, double psi_
 
 void poisson_destroy(PoissonImpl* p) {
     if (!p) return;
-    (void)hipFree(p->d_tw); (void)hipFree(p->d_kx2); (void)hipFree(p->d_cpb); (void)hipFree(p->d_gny);
+    (void)hipFree(p->d_tw); (void)hipFree(p->d_twp); (void)hipFree(p->d_kx2); (void)hipFree(p->d_cpb); (void)hipFree(p->d_gny);
     (void)hipFree(p->d_asub); (void)hipFree(p->d_csup); (void)hipFree(p->d_hat);
     delete p;
 }

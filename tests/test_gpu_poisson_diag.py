@@ -13,7 +13,34 @@ from oracle import P
 pytestmark = pytest.mark.gpu
 
 POISSON_GRIDS = [(4, 4, 2.0), (8, 9, 0.0), (64, 33, 2.0), (128, 129, 2.0), (256, 65, 0.0), (512, 257, 2.0),
-                 (2048, 129, 2.0), (8192, 33, 2.0), (16384, 17, 2.5)]
+                 (1024, 65, 2.0), (2048, 129, 2.0), (4096, 65, 1.0), (8192, 33, 2.0), (16384, 17, 2.5)]
+
+# (Nx, radix) of every instantiated register row-FFT plan, and 0 = the LDS radix-4 kernels
+FFT_PLANS = [(512, 4), (1024, 4), (1024, 8), (2048, 8), (2048, 16), (4096, 8), (4096, 16), (8192, 16),
+             (16384, 16), (4096, 0), (512, 0)]
+
+
+@pytest.mark.parametrize("Nx,radix", FFT_PLANS)
+def test_poisson_every_fft_plan(Nx, radix, monkeypatch):
+    """Each row-FFT plan (selected with NSE2D_FFT_RADIX at plan creation) on white-noise vorticity
+    with homogeneous walls, so that psi is the noise response alone and every twiddle and
+    butterfly carries weight, against the oracle."""
+    monkeypatch.setenv("NSE2D_FFT_RADIX", str(radix))
+    Ny = 33
+    print(f"[fft plan] Nx={Nx} radix={radix}", flush=True)
+    g = Grid(Nx, Ny, 2.0)
+    w = np.random.default_rng(Nx + radix).normal(size=(Ny, Nx))
+    ref = oracle.poisson_solve(w, g.Lx, g.m, 0.0, 0.0)
+    p = nh.lib().poisson2d_create(Nx, Ny, g.Lx, None)
+    assert p, nh.last_error()
+    assert nh.lib().poisson2d_enable_mapped(p, g.dp("a_node"), g.dp("a_edge"), g.inv_deta, None) == 0
+    dw, dpsi = dev(w), zeros_dev(Nx * Ny)
+    nh.lib().poisson2d_solve(p, dw.ptr, dpsi.ptr, 0.0, 0.0, 0.0, None)
+    got = dpsi.download((Ny, Nx))
+    nh.lib().poisson2d_destroy(p)
+    e = rel_err(got, ref)
+    print(f"[fft plan] Nx={Nx} radix={radix} rel err {e:.2e}", flush=True)
+    assert e <= 1e-13
 
 
 @pytest.mark.parametrize("Nx,Ny,beta", POISSON_GRIDS)
