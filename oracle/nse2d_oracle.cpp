@@ -514,8 +514,10 @@ struct OPoisson {
     const double *a_node = nullptr, *a_edge = nullptr;
     double inv_deta2 = 0;
     FFTPlan plan;
+    bool extended = false;  // solve in 80-bit arithmetic instead (measurement runs, --poisson_ld)
+    double Lx_ = 0, inv_deta_ = 0;
     OPoisson(int nx, int ny, double Lx, const double* an, const double* ae, double inv_deta)
-        : Nx(nx), Ny(ny), plan(nx) {
+        : Nx(nx), Ny(ny), plan(nx), Lx_(Lx), inv_deta_(inv_deta) {
         kx2.resize(Nx);
         orc_fill_kx2(kx2.data(), Nx, Lx);
         cp.resize((size_t)Nx * (size_t)std::max(0, Ny - 2));
@@ -524,6 +526,10 @@ struct OPoisson {
         hat.resize((size_t)2 * Nx * Ny);
     }
     void solve(const double* omega, double* psi, double psi_bot, double psi_top) {
+        if (extended) {
+            orc_poisson_solve_ld(omega, psi, Nx, Ny, Lx_, a_node, a_edge, inv_deta_, psi_bot, psi_top);
+            return;
+        }
         typedef std::complex<double> C;
         C* H = reinterpret_cast<C*>(hat.data());
 #pragma omp parallel
@@ -558,6 +564,86 @@ extern "C" void orc_poisson_solve(const double* omega, double* psi, int Nx, int 
                                   double psi_bot, double psi_top) {
     OPoisson P(Nx, Ny, Lx, a_node, a_edge, inv_deta);
     P.solve(omega, psi, psi_bot, psi_top);
+}
+
+// Extended-precision (80-bit long double) solve of the same discrete operator: coefficients as
+// the reference forms them in double (a_sub, c_sup, kx^2), FFTs and elimination in long double.
+// A measurement tool, not a restatement: it shows how far any FP64 evaluation of the solve, the
+// reference's included, is from the exact solution of the operator it defines.
+namespace {
+typedef std::complex<long double> CL;
+void fft_ld(CL* a, int N, bool inverse) {
+    for (int i = 1, j = 0; i < N; ++i) {  // bit reversal
+        int bit = N >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) std::swap(a[i], a[j]);
+    }
+    const long double pi = 3.14159265358979323846264338327950288L;
+    for (int len = 2; len <= N; len <<= 1) {
+        const int half = len >> 1;
+        for (int k = 0; k < half; ++k) {
+            const long double ang = (inverse ? 2.0L : -2.0L) * pi * (long double)k / (long double)len;
+            const CL w(std::cos(ang), std::sin(ang));
+            for (int s = 0; s < N; s += len) {
+                const CL u = a[s + k], v = a[s + k + half] * w;
+                a[s + k] = u + v;
+                a[s + k + half] = u - v;
+            }
+        }
+    }
+}
+}  // namespace
+
+extern "C" void orc_poisson_solve_ld(const double* omega, double* psi, int Nx, int Ny, double Lx,
+                                     const double* a_node, const double* a_edge, double inv_deta,
+                                     double psi_bot, double psi_top) {
+    if ((Nx & (Nx - 1)) != 0) return;
+    const int Ni = Ny - 2;
+    const double inv_deta2 = inv_deta * inv_deta;
+    std::vector<double> kx2(Nx);
+    orc_fill_kx2(kx2.data(), Nx, Lx);
+    std::vector<CL> H((size_t)Nx * Ny);
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < Ny; ++j) {
+        CL* row = H.data() + (size_t)j * Nx;
+        for (int i = 0; i < Nx; ++i) row[i] = CL(omega[ID(j, i, Nx)], 0.0L);
+        fft_ld(row, Nx, false);
+    }
+#pragma omp parallel for schedule(static)
+    for (int k = 0; k < Nx; ++k) {
+        std::vector<long double> cp(Ni);
+        std::vector<CL> dp(Ni);
+        long double cp_prev = 0.0L;
+        CL dp_prev(0.0L, 0.0L);
+        for (int r = 0; r < Ni; ++r) {
+            const int j = r + 1;
+            const double a_sub = a_node[j] * a_edge[j - 1] * inv_deta2, c_sup = a_node[j] * a_edge[j] * inv_deta2;
+            const long double b = -((long double)a_sub + (long double)c_sup) - (long double)kx2[k];
+            CL rhs = -H[(size_t)j * Nx + k];
+            if (k == 0 && r == 0) rhs -= CL((long double)a_sub * psi_bot * (long double)Nx, 0.0L);
+            if (k == 0 && r == Ni - 1) rhs -= CL((long double)c_sup * psi_top * (long double)Nx, 0.0L);
+            const long double den = b - (long double)a_sub * cp_prev;
+            cp[r] = (long double)c_sup / den;
+            dp[r] = (rhs - (long double)a_sub * dp_prev) / den;
+            cp_prev = cp[r];
+            dp_prev = dp[r];
+        }
+        CL x = dp[Ni - 1];
+        H[(size_t)Ni * Nx + k] = x;
+        for (int r = Ni - 2; r >= 0; --r) {
+            x = dp[r] - cp[r] * x;
+            H[(size_t)(r + 1) * Nx + k] = x;
+        }
+        H[k] = (k == 0 ? CL((long double)Nx * psi_bot, 0.0L) : CL(0.0L, 0.0L));
+        H[(size_t)(Ny - 1) * Nx + k] = (k == 0 ? CL((long double)Nx * psi_top, 0.0L) : CL(0.0L, 0.0L));
+    }
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < Ny; ++j) {
+        CL* row = H.data() + (size_t)j * Nx;
+        fft_ld(row, Nx, true);
+        for (int i = 0; i < Nx; ++i) psi[ID(j, i, Nx)] = (double)(row[i].real() / (long double)Nx);
+    }
 }
 
 // =============================================================================================
@@ -886,6 +972,7 @@ struct OParams {  // common.cuh:58-114
     bool free_slip{false};
     long long max_steps{-1};
     int band_kmax{32};
+    bool poisson_ld{false};  // measurement: extended-precision Poisson solves
 };
 
 struct ORun {
@@ -962,6 +1049,7 @@ void parse_args(int argc, char** argv, ORun& R) {
         else if (eq(argv[i], "--walls") && nx) P.free_slip = (std::strcmp(argv[++i], "freeslip") == 0);
         else if (eq(argv[i], "--max_steps") && nx) P.max_steps = std::atoll(argv[++i]);
         else if (eq(argv[i], "--band_kmax") && nx) P.band_kmax = std::atoi(argv[++i]);
+        else if (eq(argv[i], "--poisson_ld")) P.poisson_ld = true;
     }
     if (P.init == "laminar" && P.rand_amp > 0.0 && P.amp <= 0.0) { P.init = "rand"; P.amp = P.rand_amp; }
     if (P.init == "mms") P.mms = true;
@@ -1352,6 +1440,7 @@ struct OState {
         for (int j = 1; j < G.Ny - 1; ++j) wt_node[j] = (2.0 / (double)(G.Ny - 1)) / G.a_node[j];
         if (!init_omega_with_IC(R, w)) return false;
         poisson = new OPoisson(G.Nx, G.Ny, G.Lx, G.a_node.data(), G.a_edge.data(), inv_deta);
+        poisson->extended = P.poisson_ld;
         psi_bot = 0.0; psi_top = 2.0 * G.h * P.Ub;  // main.cu:211-213
         poisson->solve(w.data(), psi.data(), psi_bot, psi_top);
         thom(w);

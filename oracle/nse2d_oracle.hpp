@@ -76,6 +76,11 @@ void orc_poisson_tridiag_mapped(double* hat, const double* cp, const double* a_n
 void orc_poisson_solve(const double* omega, double* psi, int Nx, int Ny, double Lx,
                             const double* a_node, const double* a_edge, double inv_deta,
                             double psi_bot, double psi_top);
+// Extended-precision solve of the same operator (a measurement tool with no reference
+// counterpart): shows the distance of any FP64 evaluation from the operator's exact solution.
+void orc_poisson_solve_ld(const double* omega, double* psi, int Nx, int Ny, double Lx,
+                          const double* a_node, const double* a_edge, double inv_deta,
+                          double psi_bot, double psi_top);
 
 // ---- diag.cu reductions. Results are the raw sums before the host normalisation, as the
 // reference launchers leave them in their device scalars.

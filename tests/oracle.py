@@ -32,6 +32,7 @@ SIG = {
     "orc_precompute_cp_mapped": (None, [_vp, _vp, _vp, _vp, _i, _i, _d]),
     "orc_poisson_tridiag_mapped": (None, [_vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d]),
     "orc_poisson_solve": (None, [_vp, _vp, _i, _i, _d, _vp, _vp, _d, _d, _d]),
+    "orc_poisson_solve_ld": (None, [_vp, _vp, _i, _i, _d, _vp, _vp, _d, _d, _d]),
     "orc_enstrophy_diss_mapped": (None, [_vp, _i, _i, _d, _d, _vp, _vp, _vp]),
     "orc_enstrophy_diss_mapped_full": (None, [_vp, _i, _i, _d, _d, _vp, _vp, _vp]),
     "orc_enstrophy_diss_balance_mapped": (None, [_vp, _i, _i, _d, _d, _vp, _vp, _vp, _vp]),
@@ -109,9 +110,11 @@ def run_main(args):
     return lib().orc_run_main(len(args), argv(args))
 
 
-def poisson_solve(omega, Lx, m, psi_bot, psi_top):
+def poisson_solve(omega, Lx, m, psi_bot, psi_top, extended=False):
+    """The oracle's FP64 solve (poisson.cu restated), or with extended=True the same operator
+    solved in 80-bit arithmetic (a measurement of FP64 rounding, no reference counterpart)."""
     Ny, Nx = omega.shape
     psi = np.zeros_like(omega)
-    lib().orc_poisson_solve(P(np.ascontiguousarray(omega)), P(psi), Nx, Ny, Lx, P(m.a_node), P(m.a_edge),
-                            (Ny - 1) / 2.0, psi_bot, psi_top)
+    f = lib().orc_poisson_solve_ld if extended else lib().orc_poisson_solve
+    f(P(np.ascontiguousarray(omega)), P(psi), Nx, Ny, Lx, P(m.a_node), P(m.a_edge), (Ny - 1) / 2.0, psi_bot, psi_top)
     return psi

@@ -1,0 +1,182 @@
+"""Parity at the workload shapes.
+
+- Laminar Poiseuille channel with seeded noise, 256x257, tanh beta=2, Re_b=500 (nu=4e-3), omega0
+  = 3y + 1e-3 Gaussian noise tapered by (1-y^2)^2. At the stated fixed dt=1e-3 the run is beyond
+  the SSPRK(3,3) diffusion limit (the reference's own viscous rule gives dt_vis = 8.4e-5 here):
+  GPU and oracle diverge together and overflow within a few steps. The 2000-step comparison uses
+  dt = 8e-5 (the viscous rule's value, rounded down), where the flow stays laminar.
+- Forced turbulent channel, 4096x2049 (the benchmark workload): ten adaptive steps against the
+  oracle with the host dt rule.
+- Large grid, 16384x8193, tanh beta=2.5, band kx <= 128: the Poisson solve against the oracle
+  and one step checked stage by stage."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import harness
+import nse2d_hip as nh
+import oracle
+from gpu_util import Grid, rel_err, sync
+from oracle import P
+from test_gpu_step import OracleStepper, field, make_channel, stagewise_errors, _fmt
+
+pytestmark = pytest.mark.gpu
+
+CFG1 = ["--Nx", "256", "--Ny", "257", "--Lx", repr(2 * math.pi), "--h", "1", "--stretch", "tanh", "--beta", "2",
+        "--Ub", "1", "--Reb", "500", "--init", "lamnoise", "--amp", "1e-3", "--seed", "20241203"]
+
+
+def run_both(args, root):
+    out = {}
+    for name, solver in (("gpu", nh.run_main), ("oracle", oracle.run_main)):
+        d = os.path.join(root, name)
+        os.makedirs(d, exist_ok=True)
+        rc = solver(list(args) + ["--outdir", d])
+        assert rc == 0, (name, rc)
+        out[name] = d
+    return out["gpu"], out["oracle"]
+
+
+def ld_solver(g):
+    """The oracle's extended-precision (80-bit) solve of the same operator, as a stepper solver."""
+    return lambda w, psi_top: oracle.poisson_solve(w, g.Lx, g.m, 0.0, psi_top, extended=True)
+
+
+def test_poiseuille_noise_stated_dt_diverges_on_both(tmp_path):
+    print("[cfg 256x257] dt=1e-3 fixed", flush=True)
+    args = CFG1 + ["--dt", "1e-3", "--no-adapt", "--tend", "0.008", "--diag_save", "0.001", "--snap_save", "100",
+                   "--statsEvery", "100000"]
+    dg, do = run_both(args, str(tmp_path))
+    g, o = harness.read_diag(dg), harness.read_diag(do)
+    assert len(g) == len(o)
+    np.testing.assert_array_equal(g["step"], o["step"])
+    fin_g, fin_o = np.isfinite(g["Omega"]), np.isfinite(o["Omega"])
+    print("[cfg 256x257] Omega gpu", g["Omega"], "oracle", o["Omega"], flush=True)
+    np.testing.assert_array_equal(fin_g, fin_o)
+    assert not fin_g[-1], "expected the run to overflow at dt = 1e-3"
+    assert o["Omega"][2] > 1e3 * o["Omega"][0]  # the stiffest mode grows by orders per step
+    # before the overflow both grow the same perturbation: the rows agree to the 9 printed digits
+    # while the growth is modest, and closely until the overflow
+    for k in range(int(np.sum(fin_o))):
+        bar = 5e-8 if k < 4 else 1e-5
+        assert abs(g["Omega"][k] - o["Omega"][k]) <= bar * abs(o["Omega"][k]), k
+
+
+def test_poiseuille_noise_2000_steps(tmp_path):
+    """2000 fixed steps of dt = 8e-5: GPU driver against the oracle driver. psi is compared at
+    1e-10. For omega and the diagnostics the bar is 1e-10 or the FP64 floor of the scheme itself,
+    whichever is larger: the oracle run again with its Poisson solves in 80-bit arithmetic
+    (--poisson_ld) differs from the FP64 oracle by that floor, because the Thom closure
+    (A = 2(1/h1^2 + 1/(h1 h2)) ~ 3e6 here) amplifies the rounding of near-wall psi. The
+    diagnostics are also compared no tighter than their 9 printed digits (2e-8)."""
+    print("[cfg 256x257] dt=8e-5 fixed, 2000 steps", flush=True)
+    run = ["--dt", "8e-5", "--no-adapt", "--tend", "0.16", "--diag_save", "0.02", "--snap_save", "0.16",
+           "--statsEvery", "100000"]
+    dg, do = run_both(CFG1 + run, str(tmp_path))
+    dq = os.path.join(str(tmp_path), "oracle_ld")
+    os.makedirs(dq)
+    assert oracle.run_main(CFG1 + run + ["--poisson_ld", "--outdir", dq]) == 0
+    g, o, q = harness.read_diag(dg), harness.read_diag(do), harness.read_diag(dq)
+    assert int(g["step"][-1]) == 2000 and int(o["step"][-1]) == 2000
+    sg, so, sq = (harness.last_snap(d, 0.16) for d in (dg, do, dq))
+    e_w, e_p, e_i = rel_err(sg.omega, so.omega), rel_err(sg.psi, so.psi), rel_err(sg.omega[1:-1], so.omega[1:-1])
+    f_w, f_i = rel_err(sq.omega, so.omega), rel_err(sq.omega[1:-1], so.omega[1:-1])
+    print(f"[cfg 256x257] 2000 steps: omega {e_w:.2e} (interior {e_i:.2e}) psi {e_p:.2e}; FP64 floor omega "
+          f"{f_w:.2e} (interior {f_i:.2e})", flush=True)
+    assert e_p <= 1e-10
+    assert e_w <= max(1e-10, f_w) and e_i <= max(1e-10, f_i)
+    for col in ("K", "Omega", "eta", "epsilon", "utau_avg", "Re_tau", "tauw_bot", "tauw_top", "dOmega_rhs"):
+        scale = np.max(np.abs(o[col])) + 1e-300
+        err, floor = np.max(np.abs(g[col] - o[col])) / scale, np.max(np.abs(q[col] - o[col])) / scale
+        print(f"[cfg 256x257] {col}: {err:.2e} (FP64 floor {floor:.2e})", flush=True)
+        assert err <= max(2e-8, floor), col
+    assert np.all(np.isfinite(g["Omega"]))
+
+
+def _channel_setup(Nx, Ny, beta, Reb, band, seed):
+    g = Grid(Nx, Ny, beta)
+    args = ["--Nx", str(Nx), "--Ny", str(Ny), "--Lx", repr(2 * math.pi), "--init", "lamband", "--amp", "0.05",
+            "--band_kmax", str(band), "--stretch", "tanh", "--beta", repr(beta), "--Ub", "1", "--seed", str(seed)]
+    w0 = oracle.init_omega(args, Nx, Ny)
+    return g, w0, 2.0 / Reb
+
+
+def test_forced_channel_4096x2049_ten_adaptive_steps():
+    """Ten steps with the device dt (CFL 0.5, viscous limit) against the oracle stepping with the
+    host rule of main.cu:451-458 on its own maxima. psi at 1e-10; omega within 1e-10 or within
+    the FP64 floor of the scheme, measured by stepping the oracle with its Poisson solves in
+    80-bit arithmetic (the Thom closure coefficient is ~2e8 on this grid, so near-wall psi
+    rounding reaches the wall vorticity multiplied by that)."""
+    Nx, Ny = 4096, 2049
+    print(f"[cfg {Nx}x{Ny}] 10 adaptive steps", flush=True)
+    g, w0, nu = _channel_setup(Nx, Ny, 2.0, 2e4, 32, 20241203)
+    F0, nf, drag, cfl, cvisc, dt_max = 0.05, 12, 1e-3, 0.5, 0.5, 1e-2
+    o = OracleStepper(g, nu, F0, nf, drag, 2.0)
+    w_o = w0.copy()
+    psi_o = o.solve(w_o)
+    ch = make_channel(g, nu, F0, nf, drag, 2.0, cfl=cfl, cvisc=cvisc, dt_max=dt_max)
+    nh.upload_ptr(nh.lib().nse2d_channel_field(ch, nh.FIELD_OMEGA), w0)
+    nh.lib().nse2d_channel_init(ch)
+    dy_min = float(np.min(np.abs(g.m.dy_edge)))
+    deta = 2.0 / (Ny - 1)
+    # the FP64 floor of the scheme: the same steps with the Poisson solves in 80-bit arithmetic
+    q = OracleStepper(g, nu, F0, nf, drag, 2.0, solver=ld_solver(g))
+    w_q = w0.copy()
+    psi_q = q.solve(w_q)
+
+    def host_dt(psi):
+        m = np.zeros(3)
+        oracle.lib().orc_max_uv_from_psi_mapped(P(psi), Nx, Ny, g.inv_dx, g.inv_deta, P(g.m.a_node), P(m))
+        dt_x = cfl * (g.dx / m[0]) if m[0] > 0 else 1e30
+        dt_y = cfl * (deta / m[2]) if m[2] > 0 else 1e30
+        return min(dt_max, min(min(dt_x, dt_y), cvisc * 0.5 / (nu * (g.inv_dx2 + 1.0 / dy_min ** 2))))
+
+    t = 0.0
+    for k in range(10):
+        dt = host_dt(psi_o)
+        psi_o, w_o = o.step(psi_o, w_o, t, dt)
+        psi_q, w_q = q.step(psi_q, w_q, t, dt)
+        t += dt
+        nh.lib().nse2d_channel_step_auto(ch, 1)
+        sync()
+        dt_dev = nh.download_ptr(nh.lib().nse2d_channel_dt_ptr(ch), 1)[0]
+        assert abs(dt_dev - dt) <= 1e-14 * dt, (k, dt_dev, dt)
+    w_g, psi_g = field(ch, nh.FIELD_OMEGA, g), field(ch, nh.FIELD_PSI, g)
+    nh.lib().nse2d_channel_destroy(ch)
+    e_w, e_p, e_i = rel_err(w_g, w_o), rel_err(psi_g, psi_o), rel_err(w_g[1:-1], w_o[1:-1])
+    f_w, f_i = rel_err(w_q, w_o), rel_err(w_q[1:-1], w_o[1:-1])
+    print(f"[cfg {Nx}x{Ny}] t={t:.3e}: omega {e_w:.2e} (interior {e_i:.2e}) psi {e_p:.2e}; FP64 floor "
+          f"omega {f_w:.2e} (interior {f_i:.2e})", flush=True)
+    assert e_p <= 1e-10 and np.all(np.isfinite(w_g))
+    assert e_w <= max(1e-10, f_w) and e_i <= max(1e-10, f_i)
+
+
+def test_large_grid_16384x8193_poisson_and_stages():
+    """At 16384x8193 (8191 interior rows, beta=2.5) the serial FP64 elimination differs from an
+    extended-precision solve by ~5e-12 (measured with the oracle's elimination against 80-bit
+    arithmetic), so the solve is compared at 2e-11; the step is checked stage by stage, where both
+    sides use the same solver."""
+    Nx, Ny = 16384, 8193
+    print(f"[cfg {Nx}x{Ny}] Poisson", flush=True)
+    g, w0, nu = _channel_setup(Nx, Ny, 2.5, 1e5, 128, 20241203)
+    ref = oracle.poisson_solve(w0, g.Lx, g.m, 0.0, 2.0)
+    p = nh.lib().poisson2d_create(Nx, Ny, g.Lx, None)
+    assert p, nh.last_error()
+    assert nh.lib().poisson2d_enable_mapped(p, g.dp("a_node"), g.dp("a_edge"), g.inv_deta, None) == 0
+    dw, dpsi = nh.DeviceBuffer.from_array(w0), nh.DeviceBuffer(Nx * Ny)
+    nh.lib().poisson2d_solve(p, dw.ptr, dpsi.ptr, 0.0, 0.0, 2.0, None)
+    got = dpsi.download((Ny, Nx))
+    nh.lib().poisson2d_destroy(p)
+    dw.free()
+    dpsi.free()
+    e = rel_err(got, ref)
+    print(f"[cfg {Nx}x{Ny}] Poisson rel err {e:.2e}", flush=True)
+    assert e <= 2e-11
+    del ref, got
+    errs = stagewise_errors(Nx, Ny, None, beta=2.5, nu=nu, w0=w0)
+    print(f"[cfg {Nx}x{Ny}] stagewise: " + _fmt(errs), flush=True)
+    # the x-differences cancel more at dx = 2 pi / 16384, so the per-operator bar of 1e-13 applies
+    for k, v in errs.items():
+        assert v <= (1e-12 if k == "rhs" else 1e-13), (k, v)

@@ -152,15 +152,16 @@ def test_one_step_matches_oracle(Nx, Ny, mms):
     nh.lib().nse2d_channel_destroy(ch)
 
 
-def stagewise_errors(Nx, Ny, mms):
+def stagewise_errors(Nx, Ny, mms, beta=2.0, nu=1e-3, w0=None):
     """One fused step checked stage by stage: the oracle computes each stage's update from the
     GPU's own previous stage (omega_s from the W1/W2 fields, psi_s = the HIP solve of it), so no
     difference is carried into the Thom closure. Returns relative errors of the interior rows of
     every stage output, of its wall rows against the closure of its own psi, of the final psi and
     of the stored stage-3 RHS."""
-    g = Grid(Nx, Ny, 2.0)
-    nu, F0, nf, drag = 1e-3, 0.05, 12, 1e-3
-    w0 = channel_ic(Nx, Ny)
+    g = Grid(Nx, Ny, beta)
+    F0, nf, drag = 0.05, 12, 1e-3
+    if w0 is None:
+        w0 = channel_ic(Nx, Ny)
     solver = GpuPoisson(g)
     o = OracleStepper(g, nu, F0, nf, drag, 2.0, mms=mms, solver=solver)
     ch = make_channel(g, nu, F0, nf, drag, 2.0, mms=mms)

@@ -205,3 +205,20 @@ def test_thomas_rounding_floor_at_2049_rows():
         worst = max(worst, err)
     print(f"FP64 elimination vs extended precision at 2049 rows: {worst:.2e}")
     assert 1e-15 < worst < 2e-12
+
+
+def test_extended_precision_solve_matches_dense_and_fp64():
+    """orc_poisson_solve_ld (80-bit; used to measure the FP64 floor of whole runs) solves the same
+    operator: against the dense solve and against the FP64 solve on a small grid."""
+    Nx, Ny, Lx = 8, 9, 5.0
+    m = oracle.Metrics(Ny, 1.0, 1, 2.0)
+    w = np.random.default_rng(5).normal(size=(Ny, Nx))
+    a = oracle.poisson_solve(w, Lx, m, 0.0, 2.0)
+    b = oracle.poisson_solve(w, Lx, m, 0.0, 2.0, extended=True)
+    np.testing.assert_allclose(b, a, rtol=0, atol=1e-14 * np.max(np.abs(a)))
+    Nx, Ny = 256, 129
+    m = oracle.Metrics(Ny, 1.0, 1, 2.0)
+    w = np.random.default_rng(6).normal(size=(Ny, Nx))
+    a = oracle.poisson_solve(w, 2 * math.pi, m, 0.0, 2.0)
+    b = oracle.poisson_solve(w, 2 * math.pi, m, 0.0, 2.0, extended=True)
+    assert np.max(np.abs(a - b)) <= 1e-13 * np.max(np.abs(a))
