@@ -102,12 +102,13 @@ __global__ void __launch_bounds__(256) stage_kernel(StageArgs a) {
     }
 }
 
-// The same stage for Nx a multiple of 64: one wavefront per 64-column strip of RY rows, SY strips
+// The same stage for Nx a multiple of 64: one wavefront per 64-column strip of RY rows, SYW strips
 // stacked in y per workgroup. Each row of psi and omega_s costs one coalesced load per array plus
 // a two-lane load of the strip's halo columns (x0-1 for lane 0, x0+64 for lane 63); the x-1 and
-// x+1 neighbours come from DPP lane shifts. Row j+2 (and omega^n of row j+1) is fetched while
-// row j is computed, so every load has a full row of arithmetic to arrive.
-template <int STAGE, bool MMS, int RY, int SYW>
+// x+1 neighbours come from DPP lane shifts. Rows are fetched PF rows ahead of their use (a ring
+// of PF raw rows in registers, and of omega^n rows in stages 2, 3), so that every load has PF
+// rows of arithmetic to arrive and each wavefront keeps PF rows of requests in flight.
+template <int STAGE, int RY, int SYW, int PF>
 __global__ void __launch_bounds__(64 * SYW) stage_kernel_wave(StageArgs a) {
     const int Nx = a.Nx, Ny = a.Ny;
     const int lane = threadIdx.x;
@@ -137,7 +138,13 @@ __global__ void __launch_bounds__(64 * SYW) stage_kernel_wave(StageArgs a) {
         wm = wave_shr1(r.wc, r.wh); wc = r.wc; wp = wave_shl1(r.wc, r.wh);
     };
     Stencil9 P, W;
-    Raw nxt;
+    Raw ring[PF];     // ring[q % PF] holds row jlo + 1 + q until iteration q uses it
+    double wring[PF];  // wring[q % PF] holds omega^n of row jlo + q
+#pragma unroll
+    for (int f = 0; f < PF; ++f) {
+        if (f < RY && jlo + 1 + f < Ny) load(jlo + 1 + f, ring[f]);
+        if (STAGE > 1 && f < RY && jlo + f < jhi) wring[f] = a.wn[idx2(jlo + f, ix, Nx)];
+    }
     if (jlo > 0) {
         Raw r;
         load(jlo - 1, r);
@@ -148,16 +155,14 @@ __global__ void __launch_bounds__(64 * SYW) stage_kernel_wave(StageArgs a) {
         load(jlo, r);
         expand(r, P.xm, P.c, P.xp, W.xm, W.c, W.xp);
     }
-    if (jlo + 1 < Ny) load(jlo + 1, nxt);
-    double wn_next = (STAGE > 1) ? a.wn[idx2(jlo, ix, Nx)] : 0.0;
 #pragma unroll
     for (int q = 0; q < RY; ++q) {
         const int j = jlo + q;
         if (j >= jhi) break;
-        const Raw cur = nxt;  // row j+1
-        const double wn_c = wn_next;
-        if (q + 1 < RY && j + 2 < Ny) load(j + 2, nxt);
-        if (STAGE > 1 && q + 1 < RY && j + 1 < jhi) wn_next = a.wn[idx2(j + 1, ix, Nx)];
+        const Raw cur = ring[q % PF];  // row j+1
+        const double wn_c = (STAGE > 1) ? wring[q % PF] : 0.0;
+        if (q + PF < RY && j + 1 + PF < Ny) load(j + 1 + PF, ring[q % PF]);
+        if (STAGE > 1 && q + PF < RY && j + PF < jhi) wring[q % PF] = a.wn[idx2(j + PF, ix, Nx)];
         if (j + 1 < Ny) expand(cur, P.mp, P.yp, P.pp, W.mp, W.yp, W.pp);
         double J = 0.0, lap = 0.0;
         if (j > 0 && j < Ny - 1) {
@@ -169,8 +174,7 @@ __global__ void __launch_bounds__(64 * SYW) stage_kernel_wave(StageArgs a) {
                                    a.inv_deta2);
         }
         const double wc = W.c;
-        double rhs = -J + a.nu * lap + a.f_row[j] - a.lin_drag * wc;
-        if (MMS) rhs += mms_f_omega(a.mp, ix, a.y_node[j]);
+        const double rhs = -J + a.nu * lap + a.f_row[j] - a.lin_drag * wc;
         const size_t c = idx2(j, ix, Nx);
         if (a.rhs_out) a.rhs_out[c] = rhs;
         double o;
@@ -241,7 +245,7 @@ struct ChannelImpl {
     PoissonImpl* P = nullptr;
     double inv_dx = 0, inv_deta = 0, inv_dx2 = 0, inv_deta2 = 0, dx = 0, deta = 0;
     // optional timing of kernel classes with events on the launch stream
-    int stage_tile = 804;  // wavefront stage kernel shape RY*100 + SYW (0: generic kernel)
+    int stage_tile = 841;  // wavefront stage kernel shape RY*100 + SYW*10 + PF (0: generic kernel)
     int prof_mask = 0;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
@@ -295,38 +299,41 @@ StageArgs stage_args(ChannelImpl* c, const double* ws, const double* wn, double*
     return a;
 }
 
-// Tile shape of the wavefront stage kernel, RY rows per wavefront x SYW wavefronts per
-// workgroup; NSE2D_STAGE_TILE=RYxSYW selects another instantiated shape (used to tune on the
-// device), 0 selects the generic kernel.
+// Shape of the wavefront stage kernel: RY rows per wavefront, SYW wavefronts per workgroup and
+// prefetch depth PF, coded RY*100 + SYW*10 + PF. NSE2D_STAGE_TILE=RYxSYWxPF (or RYxSYW, PF = 2)
+// selects another instantiated shape (used to tune on the device); 0 selects the generic kernel.
+// The manufactured-solution forcing (sin/cos per point) always runs in the generic kernel, so the
+// production kernels carry none of its registers.
+constexpr int kStageDefault = 841;
 int stage_tile_code() {
-    int ry = 8, sy = 4;
+    int ry = 8, sy = 4, pf = 1;
     if (const char* e = std::getenv("NSE2D_STAGE_TILE")) {
         if (e[0] == '0') return 0;
-        if (std::sscanf(e, "%dx%d", &ry, &sy) != 2) { ry = 8; sy = 4; }
+        const int n = std::sscanf(e, "%dx%dx%d", &ry, &sy, &pf);
+        if (n == 2) pf = 2;
+        else if (n != 3) return kStageDefault;
     }
-    const int c = ry * 100 + sy;
+    const int c = ry * 100 + sy * 10 + pf;
     switch (c) {
-        case 804: case 1604: case 1602: case 1601: case 3201: case 3202: case 808: return c;
-        default: return 804;
+        case 841: case 842: case 843: case 1641: case 1642: case 1643: case 1621: case 1622: case 1623:
+        case 3211: case 3212: case 3213: return c;
+        default: return kStageDefault;
     }
 }
 
-// The manufactured-solution forcing (sin/cos per point) is a separate instantiation so that the
-// production kernels carry none of its registers.
 template <int STAGE, bool MMS>
 void launch_stage_t(ChannelImpl* c, const StageArgs& a) {
-    const int code = (c->d.Nx % 64 == 0) ? c->stage_tile : 0;
-#define NSE2D_STAGE_WAVE(RYV, SYV)                                                                 \
-    stage_kernel_wave<STAGE, MMS, RYV, SYV><<<(c->d.Nx / 64) * ceil_div(c->d.Ny, RYV * SYV),       \
-                                              dim3(64, SYV), 0, c->s>>>(a)
+    const int code = (!MMS && c->d.Nx % 64 == 0) ? c->stage_tile : 0;
+#define NSE2D_STAGE_WAVE(RYV, SYV, PFV)                                                                  \
+    case RYV * 100 + SYV * 10 + PFV:                                                                     \
+        stage_kernel_wave<STAGE, RYV, SYV, PFV><<<(c->d.Nx / 64) * ceil_div(c->d.Ny, RYV * SYV),         \
+                                                  dim3(64, SYV), 0, c->s>>>(a);                          \
+        break;
     switch (code) {
-        case 804: NSE2D_STAGE_WAVE(8, 4); break;
-        case 808: NSE2D_STAGE_WAVE(8, 8); break;
-        case 1604: NSE2D_STAGE_WAVE(16, 4); break;
-        case 1602: NSE2D_STAGE_WAVE(16, 2); break;
-        case 1601: NSE2D_STAGE_WAVE(16, 1); break;
-        case 3201: NSE2D_STAGE_WAVE(32, 1); break;
-        case 3202: NSE2D_STAGE_WAVE(32, 2); break;
+        NSE2D_STAGE_WAVE(8, 4, 1) NSE2D_STAGE_WAVE(8, 4, 2) NSE2D_STAGE_WAVE(8, 4, 3)
+        NSE2D_STAGE_WAVE(16, 4, 1) NSE2D_STAGE_WAVE(16, 4, 2) NSE2D_STAGE_WAVE(16, 4, 3)
+        NSE2D_STAGE_WAVE(16, 2, 1) NSE2D_STAGE_WAVE(16, 2, 2) NSE2D_STAGE_WAVE(16, 2, 3)
+        NSE2D_STAGE_WAVE(32, 1, 1) NSE2D_STAGE_WAVE(32, 1, 2) NSE2D_STAGE_WAVE(32, 1, 3)
         default: {
             const unsigned ntiles = ceil_div(c->d.Nx, SX) * ceil_div(c->d.Ny, SY * RY);
             stage_kernel<STAGE, MMS><<<ntiles, dim3(SX, SY), 0, c->s>>>(a);
@@ -347,6 +354,12 @@ void close_stage(ChannelImpl* c, double* f, int stage, bool bookkeeping) {
     const nse2d_channel_desc& d = c->d;
     { ProfScope ps(c, NSE2D_PROF_FFT_FWD); poisson_forward(c->P, f, c->s); }
     { ProfScope ps(c, NSE2D_PROF_THOMAS); poisson_tridiag(c->P, d.psi_bot, d.psi_top, c->s); }
+    if (!bookkeeping && poisson_fuses_closure(c->P)) {
+        // the closure rides in the inverse transform's workgroups of rows 1 and Ny-2
+        ProfScope ps(c, NSE2D_PROF_FFT_INV);
+        poisson_inverse_closure(c->P, c->psi, f, d.d_dy_edge, d.psi_bot, d.psi_top, wall_type(), c->s);
+        return;
+    }
     { ProfScope ps(c, NSE2D_PROF_FFT_INV); poisson_inverse(c->P, c->psi, c->s); }
     ProfScope ps(c, NSE2D_PROF_THOM);
     if (bookkeeping) {

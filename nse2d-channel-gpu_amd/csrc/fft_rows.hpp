@@ -117,11 +117,10 @@ __device__ inline void dft(cplx* x) {
     }
 }
 
-// Pass p >= 1 (or p = 0 of the inverse) from LDS; writes LDS, or the output row for the
-// inverse's last pass (OUT != nullptr, scaled).
+// Pass p >= 1 (or p = 0 of the inverse) from LDS; writes LDS, or, for the inverse's last pass,
+// leaves the scaled outputs in v: v[i Rp + q] is output element tid + i T + q Ns (store_row).
 template <int M, int R, bool INV, int p>
-__device__ inline void lds_pass(cplx (&v)[R], cplx* lds, const cplx* __restrict__ twp, cplx* __restrict__ out,
-                                double scale) {
+__device__ inline void lds_pass(cplx (&v)[R], cplx* lds, const cplx* __restrict__ twp, double scale) {
     using P = Plan<M, R>;
     constexpr int T = P::T, Rp = P::radix(p), Ns = P::ns(p), NI = R / Rp, MR = M / Rp;
     constexpr bool LAST = (p == P::NPASS - 1);
@@ -142,9 +141,9 @@ __device__ inline void lds_pass(cplx (&v)[R], cplx* lds, const cplx* __restrict_
         dft<Rp, INV>(v + i * Rp);
         const int base = (j - k) * Rp + k;
         if (LAST && INV) {
-            // last pass: j < Ns, so base = j and the stores are contiguous across threads
+            // last pass: j < Ns, so base = j = tid + i T; the outputs stay in registers
 #pragma unroll
-            for (int q = 0; q < Rp; ++q) out[base + q * Ns] = cscale(v[i * Rp + q], scale);
+            for (int q = 0; q < Rp; ++q) v[i * Rp + q] = cscale(v[i * Rp + q], scale);
         } else {
 #pragma unroll
             for (int q = 0; q < Rp; ++q) lds[lds_pad(base + q * Ns)] = v[i * Rp + q];
@@ -154,10 +153,19 @@ __device__ inline void lds_pass(cplx (&v)[R], cplx* lds, const cplx* __restrict_
 }
 
 template <int M, int R, bool INV, int... Ps>
-__device__ inline void lds_passes(cplx (&v)[R], cplx* lds, const cplx* twp, cplx* out, double scale,
+__device__ inline void lds_passes(cplx (&v)[R], cplx* lds, const cplx* twp, double scale,
                                   std::integer_sequence<int, Ps...>) {
-    (lds_pass<M, R, INV, Ps + (INV ? 0 : 1)>(v, lds, twp, out, scale), ...);
+    (lds_pass<M, R, INV, Ps + (INV ? 0 : 1)>(v, lds, twp, scale), ...);
 }
+
+// Output element of v[i Rp + q] after the inverse's last pass: tid + i T + q Ns. The stores of
+// one (i, q) are contiguous across the workgroup.
+template <int M, int R>
+struct LastPass {
+    using P = Plan<M, R>;
+    static constexpr int Rp = P::radix(P::NPASS - 1), Ns = P::ns(P::NPASS - 1), NI = R / Rp;
+    __device__ static int pos(int i, int q) { return (int)threadIdx.x + i * P::T + q * Ns; }
+};
 
 // Forward: real rows of omega ([rows][2M]) -> packed half spectra ([rows][M]).
 template <int M, int R>
@@ -176,7 +184,7 @@ __global__ void __launch_bounds__(M / R) rows_forward(const double* __restrict__
 #pragma unroll
     for (int q = 0; q < R; ++q) lds[lds_pad(tid * R + q)] = v[q];
     __syncthreads();
-    lds_passes<M, R, false>(v, lds, twp, nullptr, 1.0, std::make_integer_sequence<int, P::NPASS - 1>{});
+    lds_passes<M, R, false>(v, lds, twp, 1.0, std::make_integer_sequence<int, P::NPASS - 1>{});
     cplx* dst = hat + row * M;
     for (int k = tid; k <= M / 2; k += T) {
         const cplx Zk = lds[lds_pad(k)], Zm = lds[lds_pad((M - k) & (M - 1))];
@@ -193,17 +201,14 @@ __global__ void __launch_bounds__(M / R) rows_forward(const double* __restrict__
     }
 }
 
-// Inverse: packed half spectra -> real rows, scaled by `scale` (1/Nx).
+// One inverse row: packed half spectrum -> the row's M complex values (x_2n, x_2n+1), scaled by
+// `scale`, left in v (element LastPass::pos(i, q) in v[i Rp + q]). Uses all of lds.
 template <int M, int R>
-__global__ void __launch_bounds__(M / R) rows_inverse(const cplx* __restrict__ hat, double* __restrict__ psi,
-                                                      const cplx* __restrict__ twp, const cplx* __restrict__ tw,
-                                                      double scale) {
+__device__ inline void inverse_row(const cplx* __restrict__ src, cplx (&v)[R], cplx* lds,
+                                   const cplx* __restrict__ twp, const cplx* __restrict__ tw, double scale) {
     using P = Plan<M, R>;
     constexpr int T = P::T;
-    __shared__ cplx lds[P::LDS];
     const int tid = threadIdx.x;
-    const size_t row = blockIdx.x;
-    const cplx* src = hat + row * M;
     for (int k = tid; k <= M / 2; k += T) {
         if (k == 0) {
             const cplx p = src[0];  // (X[0], X[M])
@@ -218,9 +223,62 @@ __global__ void __launch_bounds__(M / R) rows_inverse(const cplx* __restrict__ h
         if (k != M - k) lds[lds_pad(M - k)] = cconj(csub(A, iWB));
     }
     __syncthreads();
+    lds_passes<M, R, true>(v, lds, twp, scale, std::make_integer_sequence<int, P::NPASS>{});
+}
+
+// Optional Thom closure of the wall rows (ops.cu:113-151), fused into the inverse transform:
+// the workgroup of row 1 also transforms row 2 (kept in registers, not stored) and writes
+// omega's bottom row; the workgroup of row Ny-2 does the same with row Ny-3 for the top row.
+struct WallClosure {
+    double* omega;           // nullptr: no closure
+    const double* dy_edge;
+    double psi_bot, psi_top;
+    int Ny, free_slip;
+};
+
+// Inverse: packed half spectra -> real rows, scaled by `scale` (1/Nx), and the wall closure.
+template <int M, int R>
+__global__ void __launch_bounds__(M / R) rows_inverse(const cplx* __restrict__ hat, double* __restrict__ psi,
+                                                      const cplx* __restrict__ twp, const cplx* __restrict__ tw,
+                                                      double scale, WallClosure wc) {
+    using P = Plan<M, R>;
+    using L = LastPass<M, R>;
+    __shared__ cplx lds[P::LDS];
+    // workgroup b -> row: 0 -> Ny-2, b in [1, Ny-2] -> b-1, Ny-1 -> Ny-1, so that the two closure
+    // workgroups (rows 1 and Ny-2, each transforming two rows) are dispatched first
+    const int b = blockIdx.x, nrow = gridDim.x;
+    const int row = (b == 0) ? nrow - 2 : (b < nrow - 1 ? b - 1 : b);
     cplx v[R];
-    cplx* dst = reinterpret_cast<cplx*>(psi + row * 2 * M);
-    lds_passes<M, R, true>(v, lds, twp, dst, scale, std::make_integer_sequence<int, P::NPASS>{});
+    inverse_row<M, R>(hat + (size_t)row * M, v, lds, twp, tw, scale);
+    cplx* dst = reinterpret_cast<cplx*>(psi + (size_t)row * 2 * M);
+#pragma unroll
+    for (int i = 0; i < L::NI; ++i)
+#pragma unroll
+        for (int q = 0; q < L::Rp; ++q) dst[L::pos(i, q)] = v[i * L::Rp + q];
+    if (!wc.omega || (row != 1 && row != wc.Ny - 2)) return;  // uniform over the workgroup
+    const bool bottom = (row == 1);
+    cplx* wrow = reinterpret_cast<cplx*>(wc.omega + (size_t)(bottom ? 0 : wc.Ny - 1) * 2 * M);
+    if (wc.free_slip) {  // ops.cu:108-111
+#pragma unroll
+        for (int i = 0; i < L::NI; ++i)
+#pragma unroll
+            for (int q = 0; q < L::Rp; ++q) wrow[L::pos(i, q)] = {0.0, 0.0};
+        return;
+    }
+    __syncthreads();  // lds is reused
+    cplx v2[R];
+    inverse_row<M, R>(hat + (size_t)(bottom ? 2 : wc.Ny - 3) * M, v2, lds, twp, tw, scale);
+    const double dy1 = wc.dy_edge[bottom ? 0 : wc.Ny - 2], dy2 = wc.dy_edge[bottom ? 1 : wc.Ny - 3];
+    const double A = 2.0 * (1.0 / (dy1 * dy1) + 1.0 / (dy1 * dy2));
+    const double B = (-2.0) * dy1 / (dy2 * (dy1 + dy2) * (dy1 + dy2));
+    const double pw = bottom ? wc.psi_bot : wc.psi_top;
+#pragma unroll
+    for (int i = 0; i < L::NI; ++i)
+#pragma unroll
+        for (int q = 0; q < L::Rp; ++q) {
+            const cplx a = v[i * L::Rp + q], b = v2[i * L::Rp + q];
+            wrow[L::pos(i, q)] = {-(A * (a.x - pw) + B * (b.x - pw)), -(A * (a.y - pw) + B * (b.y - pw))};
+        }
 }
 
 }  // namespace fft

@@ -17,5 +17,12 @@ void poisson_destroy(PoissonImpl* p);
 void poisson_forward(PoissonImpl* p, const double* omega, hipStream_t s);
 void poisson_tridiag(PoissonImpl* p, double psi_bot, double psi_top, hipStream_t s);
 void poisson_inverse(PoissonImpl* p, double* psi, hipStream_t s);
+// The inverse phase with the Thom closure of omega's wall rows fused in (ops.cu:113-151, or
+// omega_wall = 0 with free_slip). Returns false, having launched nothing, when the plan's row FFT
+// cannot fuse it (the small-row LDS kernels); the caller then runs poisson_inverse and the
+// closure kernel.
+bool poisson_fuses_closure(const PoissonImpl* p);
+bool poisson_inverse_closure(PoissonImpl* p, double* psi, double* omega, const double* dy_edge, double psi_bot,
+                             double psi_top, int free_slip, hipStream_t s);
 
 }  // namespace nse2d

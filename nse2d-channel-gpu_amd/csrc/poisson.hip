@@ -231,10 +231,17 @@ __global__ void __launch_bounds__(T) thomas_chunked_kernel(cplx* __restrict__ ha
                                                            double bc_top) {
     constexpr int NCH = T / KC;
     __shared__ double sm[2][4][T];
+    // g of the Nyquist chain (slot 0's imaginary part), staged in LDS by the one workgroup that
+    // holds slot 0 so that its serial loops do not wait on a global load per row
+    __shared__ double gq[NCH * RC];
     const int t = threadIdx.x, kc = t % KC, c = t / KC;
     const int k = blockIdx.x * KC + kc;
     const bool colok = k < M;
     const bool split = (k == 0);
+    if (blockIdx.x == 0) {
+        for (int r = t; r < Ni; r += T) gq[r] = gny[r];
+        __syncthreads();
+    }
     const int ldc = M + 1;
     const int r0 = c * RC;
     const int nr = colok ? max(0, min(RC, Ni - r0)) : 0;
@@ -258,7 +265,7 @@ __global__ void __launch_bounds__(T) thomas_chunked_kernel(cplx* __restrict__ ha
             if (split && r == 0) rx -= a * bc_bot;
             if (split && r == Ni - 1) rx -= cc * bc_top;
             const double gx = 1.0 / ((-(a + cc) - kkx) - a * cpx);
-            const double gy = split ? gny[r] : gx;
+            const double gy = split ? gq[r] : gx;
             const double cpy = cc * gy;
             cpx = cc * gx;
             dpl = {(rx - a * dpl.x) * gx, (ry - a * dpl.y) * gy};
@@ -330,7 +337,7 @@ __global__ void __launch_bounds__(T) thomas_chunked_kernel(cplx* __restrict__ ha
     for (int i = 0; i < RC; ++i) {
         if (i < nr) {
             const double a = a_sub[r0 + i];
-            const double gy = split ? gny[r0 + i] : g[i];
+            const double gy = split ? gq[r0 + i] : g[i];
             Qx *= -a * g[i];
             Qy *= -a * gy;
             f[i] = {f[i].x + Qx * D.x, f[i].y + Qy * D.y};
@@ -342,7 +349,7 @@ __global__ void __launch_bounds__(T) thomas_chunked_kernel(cplx* __restrict__ ha
         if (i < nr) {
             const int r = r0 + i;
             const double cc = c_sup[r];
-            const double gy = split ? gny[r] : g[i];
+            const double gy = split ? gq[r] : g[i];
             xn = {f[i].x - (cc * g[i]) * xn.x, f[i].y - (cc * gy) * xn.y};
             hat[(size_t)(r + 1) * M + k] = xn;
         }
@@ -427,8 +434,8 @@ void launch_rows_forward(const double* omega, cplx* hat, int rows, const cplx* t
 }
 template <int M, int R>
 void launch_rows_inverse(const cplx* hat, double* psi, int rows, const cplx* twp, const cplx* tw, double scale,
-                         hipStream_t s) {
-    fft::rows_inverse<M, R><<<rows, M / R, 0, s>>>(hat, psi, twp, tw, scale);
+                         const fft::WallClosure& wc, hipStream_t s) {
+    fft::rows_inverse<M, R><<<rows, M / R, 0, s>>>(hat, psi, twp, tw, scale, wc);
 }
 #define NSE2D_FFT_PLANS(X) X(256, 4) X(512, 4) X(512, 8) X(1024, 8) X(1024, 16) X(2048, 8) X(2048, 16) \
     X(4096, 16) X(8192, 16)
@@ -591,12 +598,31 @@ void poisson_tridiag(PoissonImpl* p, double psi_bot, double psi_top, hipStream_t
 #undef NSE2D_THOMAS
     NSE2D_LAUNCH_CHECK();
 }
+bool poisson_fuses_closure(const PoissonImpl* p) { return p->fft_radix != 0; }
+
+bool poisson_inverse_closure(PoissonImpl* p, double* psi, double* omega, const double* dy_edge, double psi_bot,
+                             double psi_top, int free_slip, hipStream_t s) {
+    if (!p->fft_radix) return false;
+    const double scale = (double)(1.0 / (double)p->Nx);
+    const fft::WallClosure wc{omega, dy_edge, psi_bot, psi_top, p->Ny, free_slip};
+    switch (p->M * 100 + p->fft_radix) {
+#define NSE2D_INVC_CASE(M, R) \
+    case M * 100 + R: launch_rows_inverse<M, R>(p->d_hat, psi, p->Ny, p->d_twp, p->d_tw, scale, wc, s); break;
+        NSE2D_FFT_PLANS(NSE2D_INVC_CASE)
+#undef NSE2D_INVC_CASE
+        default: std::abort();
+    }
+    NSE2D_LAUNCH_CHECK();
+    return true;
+}
+
 void poisson_inverse(PoissonImpl* p, double* psi, hipStream_t s) {
     const double scale = (double)(1.0 / (double)p->Nx);
     if (p->fft_radix) {
+        const fft::WallClosure none{nullptr, nullptr, 0.0, 0.0, p->Ny, 0};
         switch (p->M * 100 + p->fft_radix) {
 #define NSE2D_INV_CASE(M, R) \
-    case M * 100 + R: launch_rows_inverse<M, R>(p->d_hat, psi, p->Ny, p->d_twp, p->d_tw, scale, s); break;
+    case M * 100 + R: launch_rows_inverse<M, R>(p->d_hat, psi, p->Ny, p->d_twp, p->d_tw, scale, none, s); break;
             NSE2D_FFT_PLANS(NSE2D_INV_CASE)
 #undef NSE2D_INV_CASE
             default: std::abort();

@@ -152,7 +152,7 @@ def test_one_step_matches_oracle(Nx, Ny, mms):
     nh.lib().nse2d_channel_destroy(ch)
 
 
-def stagewise_errors(Nx, Ny, mms, beta=2.0, nu=1e-3, w0=None):
+def stagewise_errors(Nx, Ny, mms, beta=2.0, nu=1e-3, w0=None, free_slip=0):
     """One fused step checked stage by stage: the oracle computes each stage's update from the
     GPU's own previous stage (omega_s from the W1/W2 fields, psi_s = the HIP solve of it), so no
     difference is carried into the Thom closure. Returns relative errors of the interior rows of
@@ -164,7 +164,9 @@ def stagewise_errors(Nx, Ny, mms, beta=2.0, nu=1e-3, w0=None):
         w0 = channel_ic(Nx, Ny)
     solver = GpuPoisson(g)
     o = OracleStepper(g, nu, F0, nf, drag, 2.0, mms=mms, solver=solver)
+    o.fs = free_slip
     ch = make_channel(g, nu, F0, nf, drag, 2.0, mms=mms)
+    nh.lib().nse2d_set_wall_type(free_slip)
     nh.upload_ptr(nh.lib().nse2d_channel_field(ch, nh.FIELD_OMEGA), w0)
     nh.lib().nse2d_channel_init(ch)
     t, dt = 0.3, 1e-3
@@ -177,13 +179,14 @@ def stagewise_errors(Nx, Ny, mms, beta=2.0, nu=1e-3, w0=None):
     W1, W2, W3 = (field(ch, k, g) for k in (nh.FIELD_W1, nh.FIELD_W2, nh.FIELD_OMEGA))
     psi_g, rhs_g = field(ch, nh.FIELD_PSI, g), field(ch, nh.FIELD_RHS, g)
     nh.lib().nse2d_channel_destroy(ch)
+    nh.lib().nse2d_set_wall_type(0)
     L, N = oracle.lib(), w_n.size
     errs = {}
 
     def walls(name, w_gpu):
         psi_s = solver(w_gpu, 2.0)
         w_cl = w_gpu.copy()
-        L.orc_thom_wall_vorticity_mapped(P(w_cl), P(psi_s), g.Nx, g.Ny, P(g.m.dy_edge), 0.0, 2.0, 0)
+        L.orc_thom_wall_vorticity_mapped(P(w_cl), P(psi_s), g.Nx, g.Ny, P(g.m.dy_edge), 0.0, 2.0, free_slip)
         errs[name + " walls"] = rel_err(w_gpu[[0, -1]], w_cl[[0, -1]])
         return psi_s
 
@@ -217,21 +220,28 @@ def _check_stagewise(errs):
         assert v <= (1e-12 if k == "rhs" else 1e-14), (k, v)
 
 
-@pytest.mark.parametrize("Nx,Ny,mms", [(64, 33, None), (256, 129, None), (128, 65, (1.0, 0.1, 1.0, 2)),
-                                       (4096, 257, None)])
-def test_step_stagewise(Nx, Ny, mms):
+@pytest.mark.parametrize("Nx,Ny,mms,fs", [(64, 33, None, 0), (256, 129, None, 0), (128, 65, (1.0, 0.1, 1.0, 2), 0),
+                                          (512, 65, None, 0), (1024, 33, None, 0), (1024, 33, None, 1),
+                                          (2048, 65, None, 0), (4096, 257, None, 0), (4096, 4, None, 0)])
+def test_step_stagewise(Nx, Ny, mms, fs):
     """The fused step against the oracle's stage sequence (J, Laplacian, RHS, MMS, SSPRK
-    combinations, Thom; main.cu:482-538), one stage at a time."""
-    errs = stagewise_errors(Nx, Ny, mms)
+    combinations, Thom; main.cu:482-538), one stage at a time. Nx >= 512 exercises the wall
+    closure fused into the inverse transform (no-slip and free-slip; Ny = 4, where rows 1 and
+    Ny-2 are each other's partner rows); smaller rows use the separate closure kernel."""
+    errs = stagewise_errors(Nx, Ny, mms, free_slip=fs)
     print(f"[stagewise] {Nx}x{Ny}: " + _fmt(errs), flush=True)
     _check_stagewise(errs)
 
 
-@pytest.mark.parametrize("tile", ["0", "8x4", "8x8", "16x4", "16x2", "16x1", "32x1", "32x2"])
-@pytest.mark.parametrize("Nx,Ny,mms", [(256, 77, None), (128, 65, (1.0, 0.1, 1.0, 2))])
+STAGE_TILES = ["0"] + [f"{ry}x{sy}x{pf}" for ry, sy in ((8, 4), (16, 4), (16, 2), (32, 1)) for pf in (1, 2, 3)]
+
+
+@pytest.mark.parametrize("tile", STAGE_TILES)
+@pytest.mark.parametrize("Nx,Ny,mms", [(256, 77, None), (64, 21, None)])
 def test_stage_tile_shapes(tile, Nx, Ny, mms, monkeypatch):
-    """Every stage-kernel tile shape (NSE2D_STAGE_TILE at channel creation; 0 = the generic
-    kernel) on grids whose rows do not fill the last tile, stage by stage."""
+    """Every stage-kernel shape (NSE2D_STAGE_TILE=RYxSYWxPF at channel creation; 0 = the generic
+    kernel) on grids whose rows do not fill the last tile, stage by stage. The manufactured
+    forcing always runs in the generic kernel (covered by test_step_stagewise)."""
     monkeypatch.setenv("NSE2D_STAGE_TILE", tile)
     errs = stagewise_errors(Nx, Ny, mms)
     print(f"[tile {tile}] {Nx}x{Ny}: " + _fmt(errs), flush=True)
