@@ -254,6 +254,39 @@ void nse2d_channel_profile(nse2d_channel_t ch, int class_mask);
 void nse2d_channel_profile_read(nse2d_channel_t ch, double* ms, long long* count);
 
 /* ------------------------------------------------------------------------------------------
+ * 5b. Ensembles of independent channels (no reference counterpart: the reference advances one
+ *     channel per process). Members may differ in size, Re_b, forcing, initial condition and
+ *     dt; each advances with its own device-chosen dt (main.cu:440-458) exactly as
+ *     nse2d_channel_step_auto would advance it alone under the same kernel shapes. Members that
+ *     share Nx, Ny, Lx, metric arrays and wall values form a size class and share one Poisson
+ *     plan. Requirements: Nx a power of two in [512, 16384], no manufactured forcing. One step is
+ *     one grouped launch per kernel class (per size class for the transforms and the y-solve),
+ *     captured once into a hipGraph and replayed. All calls run on the ensemble's stream (a new
+ *     stream when s is NULL, since the null stream cannot be captured).
+ * ------------------------------------------------------------------------------------------ */
+typedef struct nse2d_ensemble* nse2d_ensemble_t;
+/* n member descriptors (the metric arrays are borrowed); NULL on error (nse2d_last_error). */
+nse2d_ensemble_t nse2d_ensemble_create(const nse2d_channel_desc* members, int n, hipStream_t s);
+void nse2d_ensemble_destroy(nse2d_ensemble_t e);
+int nse2d_ensemble_size(nse2d_ensemble_t e);
+/* Device pointer of member m's field (NSE2D_FIELD_OMEGA, _PSI, _W1, _W2), NULL otherwise. */
+double* nse2d_ensemble_field(nse2d_ensemble_t e, int member, int which);
+/* Device array [n][8]: member m's t, dt, max|u|, max|v|, max|v| a at 8m .. 8m+4. */
+double* nse2d_ensemble_scalars(nse2d_ensemble_t e);
+hipStream_t nse2d_ensemble_stream(nse2d_ensemble_t e);
+/* Initial solve and wall closure of every member (main.cu:211-213). */
+void nse2d_ensemble_init(nse2d_ensemble_t e);
+/* nsteps steps of every member, each with its own dt (replays of the captured step graph). */
+void nse2d_ensemble_step_auto(nse2d_ensemble_t e, int nsteps);
+/* on = 0: launch the kernels directly instead of replaying the graph. */
+void nse2d_ensemble_set_graph(nse2d_ensemble_t e, int on);
+void nse2d_ensemble_sync(nse2d_ensemble_t e);
+/* Times whole steps for each stage-kernel shape and, per size class, each row-FFT radix and
+ * Thomas column width, keeps the fastest, and restores the state it advanced while timing.
+ * Writes a one-line summary of the choice into report. Returns 0. */
+int nse2d_ensemble_autotune(nse2d_ensemble_t e, int nsteps, char* report, size_t report_len);
+
+/* ------------------------------------------------------------------------------------------
  * 6. Runtime helpers (device memory, streams) so that callers without another HIP binding
  *    (ctypes, cgo) can drive the library.
  * ------------------------------------------------------------------------------------------ */

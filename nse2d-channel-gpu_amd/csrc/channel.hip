@@ -109,11 +109,10 @@ __global__ void __launch_bounds__(256) stage_kernel(StageArgs a) {
 // of PF raw rows in registers, and of omega^n rows in stages 2, 3), so that every load has PF
 // rows of arithmetic to arrive and each wavefront keeps PF rows of requests in flight.
 template <int STAGE, int RY, int SYW, int PF>
-__global__ void __launch_bounds__(64 * SYW) stage_kernel_wave(StageArgs a) {
+__device__ inline void stage_wave_body(const StageArgs& a, int tile) {
     const int Nx = a.Nx, Ny = a.Ny;
     const int lane = threadIdx.x;
     const int ntx = Nx >> 6;
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
     const int x0 = (tile % ntx) << 6, ix = x0 + lane;
     const int jlo = ((tile / ntx) * SYW + threadIdx.y) * RY;
     if (jlo >= Ny) return;  // uniform over the wavefront
@@ -187,6 +186,23 @@ __global__ void __launch_bounds__(64 * SYW) stage_kernel_wave(StageArgs a) {
     }
 }
 
+template <int STAGE, int RY, int SYW, int PF>
+__global__ void __launch_bounds__(64 * SYW) stage_kernel_wave(StageArgs a) {
+    stage_wave_body<STAGE, RY, SYW, PF>(a, xcd_tile(blockIdx.x, gridDim.x));  // tiles contiguous per XCD
+}
+
+// The stage of several channels (ensemble members) in one launch: args[m] describes member m and
+// its tiles are the global tiles [tile_begin[m], tile_begin[m+1]); the XCD renumbering runs over
+// the global tile index, so each XCD works through one contiguous range of members' rows.
+template <int STAGE, int RY, int SYW, int PF>
+__global__ void __launch_bounds__(64 * SYW) stage_kernel_wave_group(const StageArgs* __restrict__ args,
+                                                                    const int* __restrict__ tile_begin, int n) {
+    const int b = xcd_tile(blockIdx.x, gridDim.x);
+    int m = 0;
+    while (m + 1 < n && tile_begin[m + 1] <= b) ++m;
+    stage_wave_body<STAGE, RY, SYW, PF>(args[m], b - tile_begin[m]);
+}
+
 // Forcing profile f(y_j) (ops.cu:526-531), evaluated once per row.
 __global__ void forcing_rows_kernel(double* f, const double* y, int Ny, double F0, int nforce, double h) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -219,16 +235,30 @@ __global__ void __launch_bounds__(1024) thom_bookkeeping_kernel(double* __restri
 }
 
 // dt rule of main.cu:451-458 from the maxima (umax, vmax, vmaxm) of the current psi.
-__global__ void dt_from_maxima_kernel(const double* __restrict__ m, double* __restrict__ dt_out, double dx,
-                                      double deta, double cfl, double cvisc, double nu, double inv_dx2,
-                                      double inv_dymin2, double dt_max) {
-    if (threadIdx.x != 0) return;
+struct DtRule {
+    double dx, deta, cfl, cvisc, nu, inv_dx2, inv_dymin2, dt_max;
+};
+__device__ inline double dt_rule(const double* m, const DtRule& r) {
     const double umax = m[0], vmaxm = m[2];
-    const double dt_x = (umax > 0 ? cfl * (dx / umax) : 1e30);
-    const double dt_y = (vmaxm > 0 ? cfl * (deta / vmaxm) : 1e30);
+    const double dt_x = (umax > 0 ? r.cfl * (r.dx / umax) : 1e30);
+    const double dt_y = (vmaxm > 0 ? r.cfl * (r.deta / vmaxm) : 1e30);
     const double dt_adv = fmin(dt_x, dt_y);
-    const double dt_vis = (nu > 0) ? cvisc * 0.5 / (nu * (inv_dx2 + inv_dymin2)) : 1e30;
-    *dt_out = fmin(dt_max, fmin(dt_adv, dt_vis));
+    const double dt_vis = (r.nu > 0) ? r.cvisc * 0.5 / (r.nu * (r.inv_dx2 + r.inv_dymin2)) : 1e30;
+    return fmin(r.dt_max, fmin(dt_adv, dt_vis));
+}
+__global__ void dt_from_maxima_kernel(const double* __restrict__ m, double* __restrict__ dt_out, DtRule r) {
+    if (threadIdx.x == 0) *dt_out = dt_rule(m, r);
+}
+
+// Ensembles: one thread per member; member m's scalars live at scal[8m]: t, dt, umax, vmax, vmaxm.
+constexpr int kEnsScal = 8;
+__global__ void ens_dt_kernel(const DtRule* __restrict__ rules, double* __restrict__ scal, int n) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < n) scal[(size_t)m * kEnsScal + 1] = dt_rule(scal + (size_t)m * kEnsScal + 2, rules[m]);
+}
+__global__ void ens_advance_kernel(double* __restrict__ scal, int n) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < n) scal[(size_t)m * kEnsScal] = scal[(size_t)m * kEnsScal] + scal[(size_t)m * kEnsScal + 1];
 }
 
 __global__ void advance_time_kernel(double* t, const double* dt) {
@@ -386,8 +416,402 @@ void do_step(ChannelImpl* c, double t, double dt, const double* dt_dev, int flag
 
 }  // namespace nse2d
 
+namespace nse2d {
+
+// =============================================================================================
+// Ensembles of independent channels (uneven sizes, own Re_b, IC and adaptive dt per member).
+// One step = the CFL maxima of every member (one grouped launch), every member's dt (one launch),
+// then per stage one grouped stage-kernel launch over all members and, per size class (members
+// that share Nx, Ny, Lx, metrics and wall values, hence one Poisson plan), one grouped launch
+// each of the forward transform, the Thomas solve and the inverse transform with the wall
+// closure; finally every member's time advances. The sequence is captured once into a hipGraph
+// and replayed. The grouped kernels run the single-channel kernels' bodies, so a member advances
+// exactly as it would alone under the same kernel shapes.
+// =============================================================================================
+namespace {
+
+struct EnsClass {
+    int Nx = 0, Ny = 0;
+    double Lx = 0, psi_bot = 0, psi_top = 0;
+    const double *a_node = nullptr, *a_edge = nullptr, *dy_edge = nullptr;
+    PoissonImpl* P = nullptr;
+    std::vector<int> members;
+    std::vector<void*> hats;
+    void** d_hats = nullptr;   // [nc] spectrum buffers
+    double** d_tab = nullptr;  // [4][nc]: omega, omega1, omega2, psi of the class's members
+};
+
+struct EnsMember {
+    nse2d_channel_desc d;
+    double *w = nullptr, *w1 = nullptr, *w2 = nullptr, *psi = nullptr, *f_row = nullptr;
+    double dx = 0, deta = 0, inv_dx = 0, inv_deta = 0, inv_dx2 = 0, inv_deta2 = 0;
+    int cls = 0;
+};
+
+int stage_code_ry(int c) { return c / 100; }
+int stage_code_sy(int c) { return (c / 10) % 10; }
+
+}  // namespace
+
+struct EnsembleImpl {
+    hipStream_t s = nullptr;
+    bool own_stream = false;
+    std::vector<EnsMember> mem;
+    std::vector<EnsClass> cls;
+    double* scal = nullptr;        // [n][kEnsScal]: t, dt, umax, vmax, vmaxm
+    DtRule* d_rules = nullptr;     // [n]
+    StageArgs* d_args = nullptr;   // [3][n]
+    int* d_tile_begin = nullptr;   // [n+1]
+    int total_tiles = 0;
+    MaxUVJob* d_jobs = nullptr;    // [n]
+    double* d_partial = nullptr;
+    int total_rows = 0;
+    int stage_tile = kStageDefault;
+    bool use_graph = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+void ens_drop_graph(EnsembleImpl* e) {
+    if (e->exec) (void)hipGraphExecDestroy(e->exec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    e->exec = nullptr;
+    e->graph = nullptr;
+}
+
+void ens_set_tiles(EnsembleImpl* e) {
+    const int n = (int)e->mem.size();
+    const int rows = stage_code_ry(e->stage_tile) * stage_code_sy(e->stage_tile);
+    std::vector<int> tb(n + 1, 0);
+    for (int m = 0; m < n; ++m)
+        tb[m + 1] = tb[m] + (e->mem[m].d.Nx / 64) * (int)ceil_div(e->mem[m].d.Ny, rows);
+    e->total_tiles = tb[n];
+    NSE2D_HIP_CHECK(hipMemcpyAsync(e->d_tile_begin, tb.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, e->s));
+    NSE2D_HIP_CHECK(hipStreamSynchronize(e->s));
+}
+
+void ens_launch_stage(EnsembleImpl* e, int stage) {
+    const int n = (int)e->mem.size();
+    const StageArgs* args = e->d_args + (size_t)(stage - 1) * n;
+#define NSE2D_ENS_WAVE(RYV, SYV, PFV)                                                                      \
+    case RYV * 100 + SYV * 10 + PFV:                                                                       \
+        if (stage == 1)                                                                                    \
+            stage_kernel_wave_group<1, RYV, SYV, PFV><<<e->total_tiles, dim3(64, SYV), 0, e->s>>>(args, e->d_tile_begin, n); \
+        else if (stage == 2)                                                                               \
+            stage_kernel_wave_group<2, RYV, SYV, PFV><<<e->total_tiles, dim3(64, SYV), 0, e->s>>>(args, e->d_tile_begin, n); \
+        else                                                                                               \
+            stage_kernel_wave_group<3, RYV, SYV, PFV><<<e->total_tiles, dim3(64, SYV), 0, e->s>>>(args, e->d_tile_begin, n); \
+        break;
+    switch (e->stage_tile) {
+        NSE2D_ENS_WAVE(8, 4, 1) NSE2D_ENS_WAVE(8, 4, 2) NSE2D_ENS_WAVE(8, 4, 3)
+        NSE2D_ENS_WAVE(16, 4, 1) NSE2D_ENS_WAVE(16, 4, 2) NSE2D_ENS_WAVE(16, 4, 3)
+        NSE2D_ENS_WAVE(16, 2, 1) NSE2D_ENS_WAVE(16, 2, 2) NSE2D_ENS_WAVE(16, 2, 3)
+        NSE2D_ENS_WAVE(32, 1, 1) NSE2D_ENS_WAVE(32, 1, 2) NSE2D_ENS_WAVE(32, 1, 3)
+        default: std::abort();
+    }
+#undef NSE2D_ENS_WAVE
+    NSE2D_LAUNCH_CHECK();
+}
+
+// Poisson solve and wall closure of every member's stage output: field index 1 (omega1),
+// 2 (omega2) or 0 (omega).
+void ens_close(EnsembleImpl* e, int field) {
+    for (EnsClass& c : e->cls) {
+        const int nc = (int)c.members.size();
+        double** tab = c.d_tab;
+        poisson_forward_group(c.P, tab + (size_t)field * nc, c.d_hats, nc, e->s);
+        poisson_tridiag_group(c.P, c.d_hats, nc, c.psi_bot, c.psi_top, e->s);
+        poisson_inverse_group(c.P, c.d_hats, tab + (size_t)3 * nc, tab + (size_t)field * nc, nc, c.dy_edge,
+                              c.psi_bot, c.psi_top, wall_type(), e->s);
+    }
+}
+
+void ens_record_step(EnsembleImpl* e) {
+    const int n = (int)e->mem.size();
+    launch_max_uv_group(e->d_jobs, n, e->total_rows, e->s);
+    ens_dt_kernel<<<ceil_div(n, 64), 64, 0, e->s>>>(e->d_rules, e->scal, n);
+    NSE2D_LAUNCH_CHECK();
+    ens_launch_stage(e, 1);
+    ens_close(e, 1);
+    ens_launch_stage(e, 2);
+    ens_close(e, 2);
+    ens_launch_stage(e, 3);
+    ens_close(e, 0);
+    ens_advance_kernel<<<ceil_div(n, 64), 64, 0, e->s>>>(e->scal, n);
+    NSE2D_LAUNCH_CHECK();
+}
+
+void ens_steps(EnsembleImpl* e, int nsteps) {
+    if (!e->use_graph) {
+        for (int k = 0; k < nsteps; ++k) ens_record_step(e);
+        return;
+    }
+    if (!e->exec) {
+        NSE2D_HIP_CHECK(hipStreamBeginCapture(e->s, hipStreamCaptureModeThreadLocal));
+        ens_record_step(e);
+        NSE2D_HIP_CHECK(hipStreamEndCapture(e->s, &e->graph));
+        NSE2D_HIP_CHECK(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+    }
+    for (int k = 0; k < nsteps; ++k) NSE2D_HIP_CHECK(hipGraphLaunch(e->exec, e->s));
+}
+
+void ens_destroy(EnsembleImpl* e) {
+    if (!e) return;
+    if (e->s) (void)hipStreamSynchronize(e->s);
+    ens_drop_graph(e);
+    for (EnsMember& m : e->mem)
+        for (double* f : {m.w, m.w1, m.w2, m.psi, m.f_row}) (void)hipFree(f);
+    for (EnsClass& c : e->cls) {
+        for (void* h : c.hats) (void)hipFree(h);
+        (void)hipFree(c.d_hats);
+        (void)hipFree(c.d_tab);
+        poisson_destroy(c.P);
+    }
+    for (void* q : {(void*)e->scal, (void*)e->d_rules, (void*)e->d_args, (void*)e->d_tile_begin, (void*)e->d_jobs,
+                    (void*)e->d_partial})
+        (void)hipFree(q);
+    if (e->own_stream) (void)hipStreamDestroy(e->s);
+    delete e;
+}
+
+EnsembleImpl* ens_create(const nse2d_channel_desc* descs, int n, hipStream_t s) {
+    if (!descs || n < 1) { set_last_error("nse2d_ensemble_create: no members"); return nullptr; }
+    for (int m = 0; m < n; ++m) {
+        const nse2d_channel_desc& d = descs[m];
+        if (d.Nx < 512 || d.Nx > 16384 || (d.Nx & (d.Nx - 1)) != 0 || d.Ny < 4) {
+            set_last_error("nse2d_ensemble_create: member " + std::to_string(m) +
+                           " needs Nx a power of two in [512, 16384] and Ny >= 4");
+            return nullptr;
+        }
+        if (d.mms) { set_last_error("nse2d_ensemble_create: manufactured forcing needs host time"); return nullptr; }
+        if (!d.d_a_node || !d.d_a_edge || !d.d_y_node || !d.d_dy_edge || !d.d_w_node) {
+            set_last_error("nse2d_ensemble_create: member " + std::to_string(m) + " metric arrays must be set");
+            return nullptr;
+        }
+    }
+    auto* e = new EnsembleImpl;
+    e->s = s;
+    if (!s) {  // a captured graph needs a stream other than the null stream
+        NSE2D_HIP_CHECK(hipStreamCreate(&e->s));
+        e->own_stream = true;
+    }
+    e->mem.resize(n);
+    for (int m = 0; m < n; ++m) {
+        EnsMember& M = e->mem[m];
+        M.d = descs[m];
+        const nse2d_channel_desc& d = M.d;
+        int k = 0;
+        for (; k < (int)e->cls.size(); ++k) {
+            const EnsClass& c = e->cls[k];
+            if (c.Nx == d.Nx && c.Ny == d.Ny && c.Lx == d.Lx && c.a_node == d.d_a_node && c.a_edge == d.d_a_edge &&
+                c.dy_edge == d.d_dy_edge && c.psi_bot == d.psi_bot && c.psi_top == d.psi_top)
+                break;
+        }
+        if (k == (int)e->cls.size()) {
+            EnsClass c;
+            c.Nx = d.Nx; c.Ny = d.Ny; c.Lx = d.Lx; c.psi_bot = d.psi_bot; c.psi_top = d.psi_top;
+            c.a_node = d.d_a_node; c.a_edge = d.d_a_edge; c.dy_edge = d.d_dy_edge;
+            c.P = poisson_create_with(d.Nx, d.Ny, d.Lx, -1, -1, e->s);
+            if (!c.P) { ens_destroy(e); return nullptr; }
+            e->cls.push_back(c);
+            poisson_enable_mapped(e->cls.back().P, d.d_a_node, d.d_a_edge, (double)(d.Ny - 1) / 2.0, e->s);
+        }
+        M.cls = k;
+        e->cls[k].members.push_back(m);
+        M.dx = d.Lx / (double)d.Nx;
+        M.inv_dx = 1.0 / M.dx;
+        M.inv_dx2 = M.inv_dx * M.inv_dx;
+        M.deta = 2.0 / (double)(d.Ny - 1);
+        M.inv_deta = 1.0 / M.deta;
+        M.inv_deta2 = M.inv_deta * M.inv_deta;
+        const size_t N = (size_t)d.Nx * d.Ny;
+        for (double** f : {&M.w, &M.w1, &M.w2, &M.psi}) {
+            NSE2D_HIP_CHECK(hipMalloc(f, N * sizeof(double)));
+            NSE2D_HIP_CHECK(hipMemsetAsync(*f, 0, N * sizeof(double), e->s));
+        }
+        NSE2D_HIP_CHECK(hipMalloc(&M.f_row, d.Ny * sizeof(double)));
+        forcing_rows_kernel<<<ceil_div(d.Ny, 256), 256, 0, e->s>>>(M.f_row, d.d_y_node, d.Ny, d.F0, d.nforce, d.h);
+        NSE2D_LAUNCH_CHECK();
+    }
+    for (EnsClass& c : e->cls) {
+        const int nc = (int)c.members.size();
+        std::vector<double*> tab(4 * nc);
+        for (int i = 0; i < nc; ++i) {
+            void* h = nullptr;
+            NSE2D_HIP_CHECK(hipMalloc(&h, poisson_hat_bytes(c.P)));
+            c.hats.push_back(h);
+            const EnsMember& M = e->mem[c.members[i]];
+            tab[i] = M.w; tab[nc + i] = M.w1; tab[2 * nc + i] = M.w2; tab[3 * nc + i] = M.psi;
+        }
+        NSE2D_HIP_CHECK(hipMalloc(&c.d_hats, sizeof(void*) * nc));
+        NSE2D_HIP_CHECK(hipMalloc(&c.d_tab, sizeof(double*) * 4 * nc));
+        NSE2D_HIP_CHECK(hipMemcpyAsync(c.d_hats, c.hats.data(), sizeof(void*) * nc, hipMemcpyHostToDevice, e->s));
+        NSE2D_HIP_CHECK(hipMemcpyAsync(c.d_tab, tab.data(), sizeof(double*) * 4 * nc, hipMemcpyHostToDevice, e->s));
+        NSE2D_HIP_CHECK(hipStreamSynchronize(e->s));
+    }
+    NSE2D_HIP_CHECK(hipMalloc(&e->scal, sizeof(double) * kEnsScal * n));
+    NSE2D_HIP_CHECK(hipMemsetAsync(e->scal, 0, sizeof(double) * kEnsScal * n, e->s));
+    std::vector<DtRule> rules(n);
+    std::vector<StageArgs> args(3 * (size_t)n);
+    std::vector<MaxUVJob> jobs(n);
+    int rows = 0;
+    for (int m = 0; m < n; ++m) rows += e->mem[m].d.Ny;
+    e->total_rows = rows;
+    NSE2D_HIP_CHECK(hipMalloc(&e->d_partial, sizeof(double) * 3 * (size_t)rows));
+    rows = 0;
+    for (int m = 0; m < n; ++m) {
+        const EnsMember& M = e->mem[m];
+        const nse2d_channel_desc& d = M.d;
+        rules[m] = DtRule{M.dx, M.deta, d.cfl, d.cvisc, d.nu, M.inv_dx2, 1.0 / (d.dy_min * d.dy_min), d.dt_max};
+        jobs[m] = MaxUVJob{M.psi, d.d_a_node, M.inv_dx, M.inv_deta, d.Nx, d.Ny, rows, e->d_partial + 3 * (size_t)rows,
+                           e->scal + (size_t)m * kEnsScal + 2};
+        rows += d.Ny;
+        for (int st = 1; st <= 3; ++st) {
+            StageArgs a{};
+            a.psi = M.psi;
+            a.ws = st == 1 ? M.w : (st == 2 ? M.w1 : M.w2);
+            a.wn = st == 1 ? nullptr : M.w;
+            a.out = st == 1 ? M.w1 : (st == 2 ? M.w2 : M.w);
+            a.rhs_out = nullptr;
+            a.Nx = d.Nx; a.Ny = d.Ny;
+            a.inv_dx = M.inv_dx; a.inv_deta = M.inv_deta; a.inv_dx2 = M.inv_dx2; a.inv_deta2 = M.inv_deta2;
+            a.nu = d.nu; a.lin_drag = d.lin_drag;
+            a.a_node = d.d_a_node; a.a_edge = d.d_a_edge; a.f_row = M.f_row; a.y_node = d.d_y_node;
+            a.dt_ptr = e->scal + (size_t)m * kEnsScal + 1;
+            a.dt_val = 0.0;
+            a.mms = 0;
+            a.mp = MmsParams{};
+            args[(size_t)(st - 1) * n + m] = a;
+        }
+    }
+    NSE2D_HIP_CHECK(hipMalloc(&e->d_rules, sizeof(DtRule) * n));
+    NSE2D_HIP_CHECK(hipMalloc(&e->d_args, sizeof(StageArgs) * 3 * n));
+    NSE2D_HIP_CHECK(hipMalloc(&e->d_jobs, sizeof(MaxUVJob) * n));
+    NSE2D_HIP_CHECK(hipMalloc(&e->d_tile_begin, sizeof(int) * (n + 1)));
+    NSE2D_HIP_CHECK(hipMemcpyAsync(e->d_rules, rules.data(), sizeof(DtRule) * n, hipMemcpyHostToDevice, e->s));
+    NSE2D_HIP_CHECK(hipMemcpyAsync(e->d_args, args.data(), sizeof(StageArgs) * 3 * n, hipMemcpyHostToDevice, e->s));
+    NSE2D_HIP_CHECK(hipMemcpyAsync(e->d_jobs, jobs.data(), sizeof(MaxUVJob) * n, hipMemcpyHostToDevice, e->s));
+    e->stage_tile = stage_tile_code();
+    if (e->stage_tile == 0) e->stage_tile = kStageDefault;  // the generic kernel has no group form
+    ens_set_tiles(e);
+    return e;
+}
+
+void ens_init(EnsembleImpl* e) {
+    for (EnsClass& c : e->cls) {
+        const int nc = (int)c.members.size();
+        poisson_forward_group(c.P, c.d_tab, c.d_hats, nc, e->s);
+        poisson_tridiag_group(c.P, c.d_hats, nc, c.psi_bot, c.psi_top, e->s);
+        poisson_inverse_group(c.P, c.d_hats, c.d_tab + (size_t)3 * nc, nullptr, nc, c.dy_edge, c.psi_bot, c.psi_top,
+                              0, e->s);
+    }
+    for (EnsMember& M : e->mem)
+        launch_thom(M.w, M.psi, M.d.Nx, M.d.Ny, M.d.d_dy_edge, M.d.psi_bot, M.d.psi_top, wall_type(), e->s);
+}
+
+// Mean ms per step of nsteps un-captured steps after 2 warm-up steps.
+double ens_time_steps(EnsembleImpl* e, int nsteps) {
+    ens_drop_graph(e);
+    const bool g = e->use_graph;
+    e->use_graph = false;
+    ens_steps(e, 2);
+    hipEvent_t a, b;
+    NSE2D_HIP_CHECK(hipEventCreate(&a));
+    NSE2D_HIP_CHECK(hipEventCreate(&b));
+    NSE2D_HIP_CHECK(hipEventRecord(a, e->s));
+    ens_steps(e, nsteps);
+    NSE2D_HIP_CHECK(hipEventRecord(b, e->s));
+    NSE2D_HIP_CHECK(hipEventSynchronize(b));
+    float ms = 0.f;
+    NSE2D_HIP_CHECK(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    e->use_graph = g;
+    return ms / nsteps;
+}
+
+// Coordinate search over the stage-kernel shape, then each size class's (row-FFT radix, Thomas
+// column width), timing whole ensemble steps. The state (omega, psi, t, dt) is saved before and
+// restored after, so tuning does not advance the ensemble.
+std::string ens_autotune(EnsembleImpl* e, int nsteps) {
+    std::vector<std::vector<double>> save;
+    for (EnsMember& M : e->mem) {
+        const size_t N = (size_t)M.d.Nx * M.d.Ny;
+        save.emplace_back(2 * N);
+        NSE2D_HIP_CHECK(hipMemcpyAsync(save.back().data(), M.w, N * sizeof(double), hipMemcpyDeviceToHost, e->s));
+        NSE2D_HIP_CHECK(hipMemcpyAsync(save.back().data() + N, M.psi, N * sizeof(double), hipMemcpyDeviceToHost, e->s));
+    }
+    std::vector<double> scal(kEnsScal * e->mem.size());
+    NSE2D_HIP_CHECK(hipMemcpyAsync(scal.data(), e->scal, sizeof(double) * scal.size(), hipMemcpyDeviceToHost, e->s));
+    NSE2D_HIP_CHECK(hipStreamSynchronize(e->s));
+    std::string rep;
+    {
+        const int codes[] = {841, 842, 843, 1641, 1642, 1643, 1621, 1622, 1623, 3211, 3212, 3213};
+        int best = e->stage_tile;
+        double tbest = 1e300;
+        for (int code : codes) {
+            e->stage_tile = code;
+            ens_set_tiles(e);
+            const double t = ens_time_steps(e, nsteps);
+            if (t < tbest) { tbest = t; best = code; }
+        }
+        e->stage_tile = best;
+        ens_set_tiles(e);
+        rep += "stage " + std::to_string(best / 100) + "x" + std::to_string((best / 10) % 10) + "x" +
+               std::to_string(best % 10);
+    }
+    for (EnsClass& c : e->cls) {
+        int best_r = poisson_fft_radix(c.P), best_k = poisson_thomas_kc(c.P);
+        double tbest = ens_time_steps(e, nsteps);
+        for (int r : {4, 8, 16})
+            for (int k : {8, 4, 2}) {
+                PoissonImpl* P = poisson_create_with(c.Nx, c.Ny, c.Lx, r, k, e->s);
+                if (!P) continue;
+                if (poisson_fft_radix(P) != r || poisson_thomas_kc(P) != k || !poisson_group_supported(P)) {
+                    poisson_destroy(P);
+                    continue;
+                }
+                poisson_enable_mapped(P, c.a_node, c.a_edge, (double)(c.Ny - 1) / 2.0, e->s);
+                std::swap(P, c.P);
+                const double t = ens_time_steps(e, nsteps);
+                if (t < tbest) { tbest = t; best_r = r; best_k = k; std::swap(P, c.P); }
+                std::swap(P, c.P);
+                NSE2D_HIP_CHECK(hipStreamSynchronize(e->s));
+                poisson_destroy(P);
+            }
+        if (best_r != poisson_fft_radix(c.P) || best_k != poisson_thomas_kc(c.P)) {
+            PoissonImpl* P = poisson_create_with(c.Nx, c.Ny, c.Lx, best_r, best_k, e->s);
+            poisson_enable_mapped(P, c.a_node, c.a_edge, (double)(c.Ny - 1) / 2.0, e->s);
+            NSE2D_HIP_CHECK(hipStreamSynchronize(e->s));
+            poisson_destroy(c.P);
+            c.P = P;
+        }
+        rep += "; " + std::to_string(c.Nx) + "x" + std::to_string(c.Ny) + ": fft radix " +
+               std::to_string(poisson_fft_radix(c.P)) + ", thomas KC " + std::to_string(poisson_thomas_kc(c.P));
+    }
+    ens_drop_graph(e);
+    for (size_t m = 0; m < e->mem.size(); ++m) {
+        EnsMember& M = e->mem[m];
+        const size_t N = (size_t)M.d.Nx * M.d.Ny;
+        NSE2D_HIP_CHECK(hipMemcpyAsync(M.w, save[m].data(), N * sizeof(double), hipMemcpyHostToDevice, e->s));
+        NSE2D_HIP_CHECK(hipMemcpyAsync(M.psi, save[m].data() + N, N * sizeof(double), hipMemcpyHostToDevice, e->s));
+    }
+    NSE2D_HIP_CHECK(hipMemcpyAsync(e->scal, scal.data(), sizeof(double) * scal.size(), hipMemcpyHostToDevice, e->s));
+    NSE2D_HIP_CHECK(hipStreamSynchronize(e->s));
+    return rep;
+}
+
+}  // namespace
+
+}  // namespace nse2d
+
 struct nse2d_channel {
     nse2d::ChannelImpl impl;
+};
+struct nse2d_ensemble {
+    nse2d::EnsembleImpl* impl;
 };
 
 using namespace nse2d;
@@ -476,8 +900,8 @@ void nse2d_channel_step_auto(nse2d_channel_t ch, int n) {
         {
             ProfScope ps(c, NSE2D_PROF_CFL);
             launch_max_uv(c->psi, d.Nx, d.Ny, c->inv_dx, c->inv_deta, d.d_a_node, m, m + 1, m + 2, c->s);
-            dt_from_maxima_kernel<<<1, 64, 0, c->s>>>(m, c->dt_ptr(), c->dx, c->deta, d.cfl, d.cvisc, d.nu,
-                                                      c->inv_dx2, inv_dymin2, d.dt_max);
+            dt_from_maxima_kernel<<<1, 64, 0, c->s>>>(
+                m, c->dt_ptr(), DtRule{c->dx, c->deta, d.cfl, d.cvisc, d.nu, c->inv_dx2, inv_dymin2, d.dt_max});
             NSE2D_LAUNCH_CHECK();
         }
         do_step(c, 0.0, 0.0, c->dt_ptr(), 0);
@@ -520,6 +944,45 @@ void nse2d_channel_wall_bookkeeping(nse2d_channel_t ch, double out3[3], double i
     NSE2D_HIP_CHECK(hipMemcpyAsync(raw, c->cum_ptr(), 3 * sizeof(double), hipMemcpyDeviceToHost, c->s));
     NSE2D_HIP_CHECK(hipStreamSynchronize(c->s));
     for (int k = 0; k < 3; ++k) out3[k] = raw[k] * invA_full;
+}
+
+nse2d_ensemble_t nse2d_ensemble_create(const nse2d_channel_desc* members, int n, hipStream_t s) {
+    EnsembleImpl* e = ens_create(members, n, s);
+    if (!e) return nullptr;
+    return new nse2d_ensemble{e};
+}
+void nse2d_ensemble_destroy(nse2d_ensemble_t e) {
+    if (!e) return;
+    ens_destroy(e->impl);
+    delete e;
+}
+int nse2d_ensemble_size(nse2d_ensemble_t e) { return (int)e->impl->mem.size(); }
+double* nse2d_ensemble_field(nse2d_ensemble_t e, int member, int which) {
+    if (member < 0 || member >= (int)e->impl->mem.size()) return nullptr;
+    EnsMember& M = e->impl->mem[member];
+    switch (which) {
+        case NSE2D_FIELD_OMEGA: return M.w;
+        case NSE2D_FIELD_PSI: return M.psi;
+        case NSE2D_FIELD_W1: return M.w1;
+        case NSE2D_FIELD_W2: return M.w2;
+        default: return nullptr;
+    }
+}
+double* nse2d_ensemble_scalars(nse2d_ensemble_t e) { return e->impl->scal; }
+hipStream_t nse2d_ensemble_stream(nse2d_ensemble_t e) { return e->impl->s; }
+void nse2d_ensemble_init(nse2d_ensemble_t e) { ens_init(e->impl); }
+void nse2d_ensemble_step_auto(nse2d_ensemble_t e, int nsteps) { ens_steps(e->impl, nsteps); }
+void nse2d_ensemble_set_graph(nse2d_ensemble_t e, int on) {
+    ens_drop_graph(e->impl);
+    e->impl->use_graph = on != 0;
+}
+void nse2d_ensemble_sync(nse2d_ensemble_t e) { NSE2D_HIP_CHECK(hipStreamSynchronize(e->impl->s)); }
+int nse2d_ensemble_autotune(nse2d_ensemble_t e, int nsteps, char* report, size_t report_len) {
+    const std::string r = ens_autotune(e->impl, nsteps < 1 ? 3 : nsteps);
+    if (report && report_len) {
+        std::snprintf(report, report_len, "%s", r.c_str());
+    }
+    return 0;
 }
 
 }  // extern "C"

@@ -167,16 +167,14 @@ struct LastPass {
     __device__ static int pos(int i, int q) { return (int)threadIdx.x + i * P::T + q * Ns; }
 };
 
-// Forward: real rows of omega ([rows][2M]) -> packed half spectra ([rows][M]).
+// One forward row: Nx = 2M reals -> packed half spectrum (M complex). Uses all of lds.
 template <int M, int R>
-__global__ void __launch_bounds__(M / R) rows_forward(const double* __restrict__ omega, cplx* __restrict__ hat,
-                                                      const cplx* __restrict__ twp, const cplx* __restrict__ tw) {
+__device__ inline void forward_row(const double* __restrict__ src_row, cplx* __restrict__ dst,
+                                   cplx* lds, const cplx* __restrict__ twp, const cplx* __restrict__ tw) {
     using P = Plan<M, R>;
     constexpr int T = P::T;
-    __shared__ cplx lds[P::LDS];
     const int tid = threadIdx.x;
-    const size_t row = blockIdx.x;
-    const cplx* src = reinterpret_cast<const cplx*>(omega + row * 2 * M);
+    const cplx* src = reinterpret_cast<const cplx*>(src_row);
     cplx v[R];
 #pragma unroll
     for (int q = 0; q < R; ++q) v[q] = src[tid + q * T];
@@ -185,7 +183,6 @@ __global__ void __launch_bounds__(M / R) rows_forward(const double* __restrict__
     for (int q = 0; q < R; ++q) lds[lds_pad(tid * R + q)] = v[q];
     __syncthreads();
     lds_passes<M, R, false>(v, lds, twp, 1.0, std::make_integer_sequence<int, P::NPASS - 1>{});
-    cplx* dst = hat + row * M;
     for (int k = tid; k <= M / 2; k += T) {
         const cplx Zk = lds[lds_pad(k)], Zm = lds[lds_pad((M - k) & (M - 1))];
         if (k == 0) {
@@ -199,6 +196,28 @@ __global__ void __launch_bounds__(M / R) rows_forward(const double* __restrict__
         dst[k] = cadd(E, WO);
         if (k != M - k) dst[M - k] = cconj(csub(E, WO));
     }
+}
+
+// Forward: real rows of omega ([rows][2M]) -> packed half spectra ([rows][M]).
+template <int M, int R>
+__global__ void __launch_bounds__(M / R) rows_forward(const double* __restrict__ omega, cplx* __restrict__ hat,
+                                                      const cplx* __restrict__ twp, const cplx* __restrict__ tw) {
+    __shared__ cplx lds[Plan<M, R>::LDS];
+    const size_t row = blockIdx.x;
+    forward_row<M, R>(omega + row * 2 * M, hat + row * M, lds, twp, tw);
+}
+
+// The same for a group of equally sized fields (ensemble members): workgroup b transforms row
+// b % rows of field b / rows.
+template <int M, int R>
+__global__ void __launch_bounds__(M / R) rows_forward_group(const double* const* __restrict__ omegas,
+                                                            cplx* const* __restrict__ hats, int rows,
+                                                            const cplx* __restrict__ twp,
+                                                            const cplx* __restrict__ tw) {
+    __shared__ cplx lds[Plan<M, R>::LDS];
+    const int m = blockIdx.x / rows;
+    const size_t row = blockIdx.x - m * rows;
+    forward_row<M, R>(omegas[m] + row * 2 * M, hats[m] + row * M, lds, twp, tw);
 }
 
 // One inverse row: packed half spectrum -> the row's M complex values (x_2n, x_2n+1), scaled by
@@ -236,17 +255,15 @@ struct WallClosure {
     int Ny, free_slip;
 };
 
-// Inverse: packed half spectra -> real rows, scaled by `scale` (1/Nx), and the wall closure.
+// Inverse of workgroup b of nrow (one field): packed half spectra -> real rows, scaled by `scale`
+// (1/Nx), and the wall closure. Workgroup b -> row: 0 -> Ny-2, b in [1, Ny-2] -> b-1, Ny-1 ->
+// Ny-1, so that the two closure workgroups (rows 1 and Ny-2, each transforming two rows) are
+// dispatched first.
 template <int M, int R>
-__global__ void __launch_bounds__(M / R) rows_inverse(const cplx* __restrict__ hat, double* __restrict__ psi,
-                                                      const cplx* __restrict__ twp, const cplx* __restrict__ tw,
-                                                      double scale, WallClosure wc) {
-    using P = Plan<M, R>;
+__device__ inline void inverse_block(const cplx* __restrict__ hat, double* __restrict__ psi, cplx* lds,
+                                     const cplx* __restrict__ twp, const cplx* __restrict__ tw, double scale,
+                                     const WallClosure& wc, int b, int nrow) {
     using L = LastPass<M, R>;
-    __shared__ cplx lds[P::LDS];
-    // workgroup b -> row: 0 -> Ny-2, b in [1, Ny-2] -> b-1, Ny-1 -> Ny-1, so that the two closure
-    // workgroups (rows 1 and Ny-2, each transforming two rows) are dispatched first
-    const int b = blockIdx.x, nrow = gridDim.x;
     const int row = (b == 0) ? nrow - 2 : (b < nrow - 1 ? b - 1 : b);
     cplx v[R];
     inverse_row<M, R>(hat + (size_t)row * M, v, lds, twp, tw, scale);
@@ -276,9 +293,32 @@ __global__ void __launch_bounds__(M / R) rows_inverse(const cplx* __restrict__ h
     for (int i = 0; i < L::NI; ++i)
 #pragma unroll
         for (int q = 0; q < L::Rp; ++q) {
-            const cplx a = v[i * L::Rp + q], b = v2[i * L::Rp + q];
-            wrow[L::pos(i, q)] = {-(A * (a.x - pw) + B * (b.x - pw)), -(A * (a.y - pw) + B * (b.y - pw))};
+            const cplx a = v[i * L::Rp + q], c = v2[i * L::Rp + q];
+            wrow[L::pos(i, q)] = {-(A * (a.x - pw) + B * (c.x - pw)), -(A * (a.y - pw) + B * (c.y - pw))};
         }
+}
+
+template <int M, int R>
+__global__ void __launch_bounds__(M / R) rows_inverse(const cplx* __restrict__ hat, double* __restrict__ psi,
+                                                      const cplx* __restrict__ twp, const cplx* __restrict__ tw,
+                                                      double scale, WallClosure wc) {
+    __shared__ cplx lds[Plan<M, R>::LDS];
+    inverse_block<M, R>(hat, psi, lds, twp, tw, scale, wc, blockIdx.x, gridDim.x);
+}
+
+// The same for a group of equally sized fields: workgroup b handles block b % rows of field
+// b / rows; omegas (may be null) gives each field's vorticity for the closure.
+template <int M, int R>
+__global__ void __launch_bounds__(M / R) rows_inverse_group(const cplx* const* __restrict__ hats,
+                                                            double* const* __restrict__ psis,
+                                                            double* const* __restrict__ omegas, int rows,
+                                                            const cplx* __restrict__ twp,
+                                                            const cplx* __restrict__ tw, double scale,
+                                                            WallClosure wc) {
+    __shared__ cplx lds[Plan<M, R>::LDS];
+    const int m = blockIdx.x / rows;
+    wc.omega = omegas ? omegas[m] : nullptr;
+    inverse_block<M, R>(hats[m], psis[m], lds, twp, tw, scale, wc, blockIdx.x - m * rows, rows);
 }
 
 }  // namespace fft

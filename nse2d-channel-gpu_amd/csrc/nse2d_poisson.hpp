@@ -7,6 +7,11 @@ namespace nse2d {
 struct PoissonImpl;
 
 PoissonImpl* poisson_create(int Nx, int Ny, double Lx, hipStream_t s);
+// With an explicit row-FFT radix (0 = LDS radix-4 kernels) and Thomas column width; -1 keeps the
+// default (or the NSE2D_FFT_RADIX / NSE2D_THOMAS_KC setting). Unavailable choices fall back.
+PoissonImpl* poisson_create_with(int Nx, int Ny, double Lx, int fft_radix, int thomas_kc, hipStream_t s);
+int poisson_fft_radix(const PoissonImpl* p);
+int poisson_thomas_kc(const PoissonImpl* p);
 int poisson_enable_mapped(PoissonImpl* p, const double* a_node, const double* a_edge,
                           double inv_deta, hipStream_t s);
 // omega -> psi; psi = psi_bot / psi_top on the wall rows. omega and psi must not alias.
@@ -24,5 +29,20 @@ void poisson_inverse(PoissonImpl* p, double* psi, hipStream_t s);
 bool poisson_fuses_closure(const PoissonImpl* p);
 bool poisson_inverse_closure(PoissonImpl* p, double* psi, double* omega, const double* dy_edge, double psi_bot,
                              double psi_top, int free_slip, hipStream_t s);
+
+// Group launches over n equally sized fields (ensemble members) that share plan p's coefficients
+// (same Nx, Ny, Lx and metrics); each field has its own spectrum buffer of poisson_hat_bytes(p).
+// The pointer tables are device arrays of n pointers. Available when poisson_group_supported(p)
+// (the register row FFT, Nx in [512, 16384]). The inverse fuses the wall closure of each field's
+// omega (d_omegas may be null: no closure).
+size_t poisson_hat_bytes(const PoissonImpl* p);
+int poisson_group_supported(const PoissonImpl* p);
+void poisson_forward_group(PoissonImpl* p, const double* const* d_omegas, void* const* d_hats, int n,
+                           hipStream_t s);
+void poisson_tridiag_group(PoissonImpl* p, void* const* d_hats, int n, double psi_bot, double psi_top,
+                           hipStream_t s);
+void poisson_inverse_group(PoissonImpl* p, void* const* d_hats, double* const* d_psis, double* const* d_omegas,
+                           int n, const double* dy_edge, double psi_bot, double psi_top, int free_slip,
+                           hipStream_t s);
 
 }  // namespace nse2d

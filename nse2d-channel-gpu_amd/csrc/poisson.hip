@@ -222,23 +222,27 @@ __global__ void column_g_kernel(double* __restrict__ gcol, const double* __restr
 // Real and imaginary parts carry their own coefficient chains; they differ only in slot 0 (real:
 // kx = 0, imaginary: Nyquist), whose second chain reads its precomputed g (gny).
 template <int RC, int KC, int T>
-__global__ void __launch_bounds__(T) thomas_chunked_kernel(cplx* __restrict__ hat, int M, int Ni,
-                                                           const double* __restrict__ a_sub,
-                                                           const double* __restrict__ c_sup,
-                                                           const double* __restrict__ kx2,
-                                                           const double* __restrict__ cpb,
-                                                           const double* __restrict__ gny, double bc_bot,
-                                                           double bc_top) {
-    constexpr int NCH = T / KC;
-    __shared__ double sm[2][4][T];
+struct ThomasShared {
+    double sm[2][4][T];
     // g of the Nyquist chain (slot 0's imaginary part), staged in LDS by the one workgroup that
     // holds slot 0 so that its serial loops do not wait on a global load per row
-    __shared__ double gq[NCH * RC];
+    double gq[(T / KC) * RC];
+};
+
+// The solve of column group `grp` (columns grp*KC .. grp*KC+KC-1) of one spectrum.
+template <int RC, int KC, int T>
+__device__ inline void thomas_group(ThomasShared<RC, KC, T>& shm, int grp, cplx* __restrict__ hat, int M, int Ni,
+                                    const double* __restrict__ a_sub, const double* __restrict__ c_sup,
+                                    const double* __restrict__ kx2, const double* __restrict__ cpb,
+                                    const double* __restrict__ gny, double bc_bot, double bc_top) {
+    constexpr int NCH = T / KC;
+    auto& sm = shm.sm;
+    auto& gq = shm.gq;
     const int t = threadIdx.x, kc = t % KC, c = t / KC;
-    const int k = blockIdx.x * KC + kc;
+    const int k = grp * KC + kc;
     const bool colok = k < M;
     const bool split = (k == 0);
-    if (blockIdx.x == 0) {
+    if (grp == 0) {
         for (int r = t; r < Ni; r += T) gq[r] = gny[r];
         __syncthreads();
     }
@@ -360,6 +364,34 @@ __global__ void __launch_bounds__(T) thomas_chunked_kernel(cplx* __restrict__ ha
     }
 }
 
+template <int RC, int KC, int T>
+__global__ void __launch_bounds__(T) thomas_chunked_kernel(cplx* __restrict__ hat, int M, int Ni,
+                                                           const double* __restrict__ a_sub,
+                                                           const double* __restrict__ c_sup,
+                                                           const double* __restrict__ kx2,
+                                                           const double* __restrict__ cpb,
+                                                           const double* __restrict__ gny, double bc_bot,
+                                                           double bc_top) {
+    __shared__ ThomasShared<RC, KC, T> S;
+    thomas_group<RC, KC, T>(S, blockIdx.x, hat, M, Ni, a_sub, c_sup, kx2, cpb, gny, bc_bot, bc_top);
+}
+
+// A group of equally sized spectra sharing one plan: workgroup b solves column group
+// b % groups of spectrum b / groups.
+template <int RC, int KC, int T>
+__global__ void __launch_bounds__(T) thomas_chunked_group_kernel(cplx* const* __restrict__ hats, int groups,
+                                                                 int M, int Ni, const double* __restrict__ a_sub,
+                                                                 const double* __restrict__ c_sup,
+                                                                 const double* __restrict__ kx2,
+                                                                 const double* __restrict__ cpb,
+                                                                 const double* __restrict__ gny, double bc_bot,
+                                                                 double bc_top) {
+    __shared__ ThomasShared<RC, KC, T> S;
+    const int m = blockIdx.x / groups;
+    thomas_group<RC, KC, T>(S, blockIdx.x - m * groups, hats[m], M, Ni, a_sub, c_sup, kx2, cpb, gny, bc_bot,
+                            bc_top);
+}
+
 int ilog2(int n) {
     int l = 0;
     while ((1 << l) < n) ++l;
@@ -399,7 +431,8 @@ bool fft_instantiated(int M, int R) {
         default: return false;
     }
 }
-int choose_fft_radix(int M) {
+int choose_fft_radix(int M, int want) {
+    if (want >= 0 && (want == 0 || fft_instantiated(M, want))) return want;
     if (const char* e = std::getenv("NSE2D_FFT_RADIX")) {
         const int r = std::atoi(e);
         if (r == 0 || fft_instantiated(M, r)) return r;
@@ -451,10 +484,11 @@ bool thomas_instantiated(int kc, int rc) {
     if (kc == 2) return rc == 8 || rc == 16 || rc == 32;
     return false;
 }
-void choose_thomas_layout(PoissonImpl* p) {
+void choose_thomas_layout(PoissonImpl* p, int want) {
     const int Ni = p->Ny - 2;
     int pref = 8;
     if (const char* e = std::getenv("NSE2D_THOMAS_KC")) pref = std::atoi(e);
+    if (want > 0) pref = want;
     for (int kc : {pref, 8, 4, 2}) {
         if (kc != 8 && kc != 4 && kc != 2) continue;
         const int nch = kThomasThreads / kc;
@@ -468,6 +502,10 @@ void choose_thomas_layout(PoissonImpl* p) {
 }  // namespace
 
 PoissonImpl* poisson_create(int Nx, int Ny, double Lx, hipStream_t s) {
+    return poisson_create_with(Nx, Ny, Lx, -1, -1, s);
+}
+
+PoissonImpl* poisson_create_with(int Nx, int Ny, double Lx, int fft_radix, int thomas_kc, hipStream_t s) {
     if (Nx < 4 || Nx > 16384 || (Nx & (Nx - 1)) != 0) {
         set_last_error("poisson2d_create: Nx must be a power of two in [4, 16384], got " + std::to_string(Nx));
         return nullptr;
@@ -479,7 +517,7 @@ PoissonImpl* poisson_create(int Nx, int Ny, double Lx, hipStream_t s) {
     auto* p = new PoissonImpl;
     p->Nx = Nx; p->Ny = Ny; p->Lx = Lx;
     p->M = Nx / 2;
-    choose_thomas_layout(p);
+    choose_thomas_layout(p, thomas_kc);
     if (p->kc == 0) {
         set_last_error("poisson2d_create: Ny = " + std::to_string(Ny) + " exceeds the y-solve layouts");
         delete p;
@@ -518,7 +556,7 @@ PoissonImpl* poisson_create(int Nx, int Ny, double Lx, hipStream_t s) {
         const double kphys = (2.0 * kPi * (double)k) / Lx;
         kx2[k] = kphys * kphys;
     }
-    p->fft_radix = choose_fft_radix(p->M);
+    p->fft_radix = choose_fft_radix(p->M, fft_radix);
     if (p->fft_radix) {
         const std::vector<cplx> twp = fft_pass_twiddles(p->M, p->fft_radix);
         NSE2D_HIP_CHECK(hipMalloc(&p->d_twp, sizeof(cplx) * std::max<size_t>(1, twp.size())));
@@ -634,6 +672,83 @@ void poisson_inverse(PoissonImpl* p, double* psi, hipStream_t s) {
         fft_rows_inverse_kernel<1><<<p->Ny, p->threads, p->lds_bytes, s>>>(p->d_hat, psi, p->Nx, p->log2M, p->d_tw, scale);
     else
         fft_rows_inverse_kernel<2><<<p->Ny, p->threads, p->lds_bytes, s>>>(p->d_hat, psi, p->Nx, p->log2M, p->d_tw, scale);
+    NSE2D_LAUNCH_CHECK();
+}
+
+// ---- group launches: n equally sized fields that share plan p's coefficients, each with its own
+// spectrum buffer (poisson_hat_bytes). Pointer tables are device arrays of n pointers.
+size_t poisson_hat_bytes(const PoissonImpl* p) { return sizeof(cplx) * (size_t)p->Ny * p->M; }
+
+template <int M, int R>
+void launch_rows_forward_group(const double* const* om, cplx* const* hats, int n, int rows, const cplx* twp,
+                               const cplx* tw, hipStream_t s) {
+    fft::rows_forward_group<M, R><<<n * rows, M / R, 0, s>>>(om, hats, rows, twp, tw);
+}
+template <int M, int R>
+void launch_rows_inverse_group(const cplx* const* hats, double* const* psis, double* const* om, int n, int rows,
+                               const cplx* twp, const cplx* tw, double scale, const fft::WallClosure& wc,
+                               hipStream_t s) {
+    fft::rows_inverse_group<M, R><<<n * rows, M / R, 0, s>>>(hats, psis, om, rows, twp, tw, scale, wc);
+}
+
+int poisson_group_supported(const PoissonImpl* p) { return p->fft_radix != 0; }
+int poisson_fft_radix(const PoissonImpl* p) { return p->fft_radix; }
+int poisson_thomas_kc(const PoissonImpl* p) { return p->kc; }
+
+void poisson_forward_group(PoissonImpl* p, const double* const* d_omegas, void* const* d_hats, int n,
+                           hipStream_t s) {
+    cplx* const* hats = reinterpret_cast<cplx* const*>(d_hats);
+    switch (p->M * 100 + p->fft_radix) {
+#define NSE2D_FWDG_CASE(M, R) \
+    case M * 100 + R: launch_rows_forward_group<M, R>(d_omegas, hats, n, p->Ny, p->d_twp, p->d_tw, s); break;
+        NSE2D_FFT_PLANS(NSE2D_FWDG_CASE)
+#undef NSE2D_FWDG_CASE
+        default: std::abort();
+    }
+    NSE2D_LAUNCH_CHECK();
+}
+
+void poisson_tridiag_group(PoissonImpl* p, void* const* d_hats, int n, double psi_bot, double psi_top,
+                           hipStream_t s) {
+    cplx* const* hats = reinterpret_cast<cplx* const*>(d_hats);
+    const int Ni = p->Ny - 2;
+    const double bb = (double)p->Nx * psi_bot, bt = (double)p->Nx * psi_top;
+    const int groups = (int)ceil_div(p->M, p->kc);
+#define NSE2D_THOMAS_G(RC, KC)                                                                           \
+    thomas_chunked_group_kernel<RC, KC, kThomasThreads><<<n * groups, kThomasThreads, 0, s>>>(           \
+        hats, groups, p->M, Ni, p->d_asub, p->d_csup, p->d_kx2, p->d_cpb, p->d_gny, bb, bt)
+    switch (p->kc * 100 + p->rc) {
+        case 801: NSE2D_THOMAS_G(1, 8); break;
+        case 802: NSE2D_THOMAS_G(2, 8); break;
+        case 804: NSE2D_THOMAS_G(4, 8); break;
+        case 808: NSE2D_THOMAS_G(8, 8); break;
+        case 816: NSE2D_THOMAS_G(16, 8); break;
+        case 832: NSE2D_THOMAS_G(32, 8); break;
+        case 416: NSE2D_THOMAS_G(16, 4); break;
+        case 432: NSE2D_THOMAS_G(32, 4); break;
+        case 208: NSE2D_THOMAS_G(8, 2); break;
+        case 216: NSE2D_THOMAS_G(16, 2); break;
+        default: NSE2D_THOMAS_G(32, 2); break;
+    }
+#undef NSE2D_THOMAS_G
+    NSE2D_LAUNCH_CHECK();
+}
+
+void poisson_inverse_group(PoissonImpl* p, void* const* d_hats, double* const* d_psis, double* const* d_omegas,
+                           int n, const double* dy_edge, double psi_bot, double psi_top, int free_slip,
+                           hipStream_t s) {
+    const cplx* const* hats = reinterpret_cast<const cplx* const*>(d_hats);
+    const double scale = (double)(1.0 / (double)p->Nx);
+    const fft::WallClosure wc{nullptr, dy_edge, psi_bot, psi_top, p->Ny, free_slip};
+    switch (p->M * 100 + p->fft_radix) {
+#define NSE2D_INVG_CASE(M, R)                                                                             \
+    case M * 100 + R:                                                                                     \
+        launch_rows_inverse_group<M, R>(hats, d_psis, d_omegas, n, p->Ny, p->d_twp, p->d_tw, scale, wc, s); \
+        break;
+        NSE2D_FFT_PLANS(NSE2D_INVG_CASE)
+#undef NSE2D_INVG_CASE
+        default: std::abort();
+    }
     NSE2D_LAUNCH_CHECK();
 }
 

@@ -77,6 +77,18 @@ SIGNATURES = {
     "nse2d_channel_wall_bookkeeping": (None, [_vp, _p, _d]),
     "nse2d_channel_profile": (None, [_vp, _i]),
     "nse2d_channel_profile_read": (None, [_vp, _p, ctypes.POINTER(ctypes.c_longlong)]),
+    # ensembles
+    "nse2d_ensemble_create": (_vp, [_vp, _i, _stream]),
+    "nse2d_ensemble_destroy": (None, [_vp]),
+    "nse2d_ensemble_size": (_i, [_vp]),
+    "nse2d_ensemble_field": (_vp, [_vp, _i, _i]),
+    "nse2d_ensemble_scalars": (_vp, [_vp]),
+    "nse2d_ensemble_stream": (_vp, [_vp]),
+    "nse2d_ensemble_init": (None, [_vp]),
+    "nse2d_ensemble_step_auto": (None, [_vp, _i]),
+    "nse2d_ensemble_set_graph": (None, [_vp, _i]),
+    "nse2d_ensemble_sync": (None, [_vp]),
+    "nse2d_ensemble_autotune": (_i, [_vp, _i, ctypes.c_char_p, _sz]),
     "nse2d_host_metrics": (_i, [_i, _d, ctypes.c_char_p, _d, _vp, _vp, _vp, _vp, _vp]),
     "nse2d_host_init_omega": (_i, [_i, ctypes.POINTER(ctypes.c_char_p), _vp]),
     # runtime
