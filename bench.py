@@ -112,8 +112,104 @@ def cpu_baseline(c, budget_s):
                       f"warm step, FP64 OpenMP oracle, {el:.2f} s"}
 
 
+# Batched ensemble: 64 independent channels of uneven size (32 at 512x257, 24 at 1024x513, 8 at
+# 2048x1025), tanh beta = 2, Re_b log-uniform in [2e3, 1e4] (seeded), own IC seed each, forcing and
+# drag as the single channel, adaptive dt per member.
+ENSEMBLE = [(512, 257, 32), (1024, 513, 24), (2048, 1025, 8)]
+
+
+def ensemble_members(seed=20241203):
+    rng = np.random.default_rng(seed)
+    out = []
+    for Nx, Ny, count in ENSEMBLE:
+        for _ in range(count):
+            out.append((Nx, Ny, float(np.exp(rng.uniform(np.log(2e3), np.log(1e4)))), 1000 + len(out)))
+    return out
+
+
+def run_ensemble(a):
+    """One line of JSON: aggregate grid-point steps/s of the ensemble (autotuned, graph-replayed
+    steps) and of the same members advanced one after another as single channels."""
+    L = nh.lib()
+    members = ensemble_members()
+    grids, keep, descs, ics = {}, [], [], []
+    for Nx, Ny, Reb, seed in members:
+        c = dict(CFG, Nx=Nx, Ny=Ny, Reb=Reb, seed=seed)
+        if (Nx, Ny) not in grids:
+            m = {k: np.zeros(Ny if k in ("y", "a_node", "w_node") else Ny - 1) for k in
+                 ("y", "a_node", "a_edge", "dy_edge", "w_node")}
+            if L.nse2d_host_metrics(Ny, 1.0, b"tanh", c["beta"], *[m[k].ctypes.data for k in
+                                    ("y", "a_node", "a_edge", "dy_edge", "w_node")]):
+                raise RuntimeError(nh.last_error())
+            grids[(Nx, Ny)] = (m, {k: nh.DeviceBuffer.from_array(v) for k, v in m.items()})
+        m, dm = grids[(Nx, Ny)]
+        w0 = np.zeros(Nx * Ny)
+        args = ic_args(c)
+        if L.nse2d_host_init_omega(len(args), cargv(args), w0.ctypes.data):
+            raise RuntimeError(nh.last_error())
+        d = nh.ChannelDesc()
+        d.Nx, d.Ny, d.Lx, d.h = Nx, Ny, c["Lx"], c["h"]
+        d.nu, d.F0, d.lin_drag, d.nforce = 2.0 * c["h"] * c["Ub"] / Reb, c["F0"], c["drag"], c["nforce"]
+        d.psi_bot, d.psi_top = 0.0, 2.0 * c["h"] * c["Ub"]
+        d.cfl, d.cvisc, d.dt_max = c["cfl"], c["cvisc"], c["dt_max"]
+        d.dy_min = float(np.min(np.abs(m["dy_edge"])))
+        d.d_a_node, d.d_a_edge, d.d_y_node = dm["a_node"].ptr, dm["a_edge"].ptr, dm["y"].ptr
+        d.d_dy_edge, d.d_w_node = dm["dy_edge"].ptr, dm["w_node"].ptr
+        descs.append(d)
+        ics.append(w0)
+    pts = sum(Nx * Ny for Nx, Ny, _, _ in members)
+    # the members one after another as single channels (same shapes as the untuned ensemble)
+    serial_s, serial_steps = 0.0, max(10, a.steps // 10)
+    for d, w0 in zip(descs, ics):
+        ch = L.nse2d_channel_create(ctypes.byref(d), None)
+        nh.upload_ptr(L.nse2d_channel_field(ch, nh.FIELD_OMEGA), w0)
+        L.nse2d_channel_init(ch)
+        L.nse2d_channel_step_auto(ch, 5)
+        L.nse2d_device_sync()
+        t0 = time.perf_counter()
+        L.nse2d_channel_step_auto(ch, serial_steps)
+        L.nse2d_device_sync()
+        serial_s += (time.perf_counter() - t0) / serial_steps
+        L.nse2d_channel_destroy(ch)
+    arr = (nh.ChannelDesc * len(descs))(*descs)
+    e = L.nse2d_ensemble_create(arr, len(descs), None)
+    if not e:
+        raise RuntimeError(nh.last_error())
+    for k, w0 in enumerate(ics):
+        nh.upload_ptr(L.nse2d_ensemble_field(e, k, nh.FIELD_OMEGA), w0)
+    L.nse2d_ensemble_init(e)
+    rep = ctypes.create_string_buffer(1024)
+    L.nse2d_ensemble_autotune(e, 3, rep, 1024)
+    L.nse2d_ensemble_step_auto(e, a.warmup)
+    L.nse2d_ensemble_sync(e)
+    t0 = time.perf_counter()
+    L.nse2d_ensemble_step_auto(e, a.steps)
+    L.nse2d_ensemble_sync(e)
+    el = time.perf_counter() - t0
+    sc = nh.download_ptr(L.nse2d_ensemble_scalars(e), 8 * len(descs)).reshape(-1, 8)
+    if not np.all(np.isfinite(sc[:, :2])):
+        raise RuntimeError("non-finite member time or dt")
+    L.nse2d_ensemble_destroy(e)
+    value = pts * a.steps / el
+    serial = pts / serial_s
+    out = {
+        "metric": "aggregate grid-point·steps/s, batched ensemble of 64 channels FP64", "value": value,
+        "unit": UNIT, "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": el / a.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "ensemble: 32 x 512x257 + 24 x 1024x513 + 8 x 2048x1025, tanh beta=2, Re_b "
+                               "log-uniform [2e3, 1e4], own seed and adaptive dt per member",
+                   "members": len(descs), "grid_points": pts},
+        "serial_single_channel_value": serial, "speedup_vs_serial": value / serial,
+        "step_roofline_frac": 240.0 * value / (HBM_PEAK_GBS * 1e9),
+        "autotune": rep.value.decode(), "member_t_range": [float(sc[:, 0].min()), float(sc[:, 0].max())],
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", choices=["channel", "ensemble"], default="channel",
+                    help="channel: the 4096x2049 forced channel (default); ensemble: 64 independent channels")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=200)
@@ -137,6 +233,9 @@ def main():
         L.nse2d_device_sync()
         src.free()
         dst.free()
+    if a.workload == "ensemble":
+        run_ensemble(a)
+        return
     c = CFG
     Nx, Ny = c["Nx"], c["Ny"]
     ch, keep = setup_channel(c)

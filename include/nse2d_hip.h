@@ -129,6 +129,23 @@ void poisson2d_profile_get(nse2d_poisson_t p, float* fft_ms, float* tridiag_ms);
 /* poisson.cuh:56-63 (destructor) */
 void poisson2d_destroy(nse2d_poisson_t p);
 
+/* x-spectra of u and v, time-averaged over the calls to accumulate: replaces struct Spectra1D
+ * (diag.cuh:123-191; kernels diag.cu:869-898) and its cuFFT D2Z plan. Per call, for
+ * k = 0..Nx/2: E(k) += wx / (Nx^2 Ny) sum_j |X_j(k)|^2 with wx = 2 for 0 < k < Nx/2, else 1, so
+ * that sum_k E_u(k) is the mean of u^2 over the grid points. Same size limits as the Poisson
+ * plan. */
+typedef struct nse2d_spectra1d* nse2d_spectra1d_t;
+/* diag.cuh:133-154 (init) */
+nse2d_spectra1d_t nse2d_spectra1d_create(int Nx, int Ny, double Lx, hipStream_t s);
+/* diag.cuh:165-174 (accumulate): d_u, d_v are [Ny][Nx] device fields */
+void nse2d_spectra1d_accumulate(nse2d_spectra1d_t sp, const double* d_u, const double* d_v,
+                                hipStream_t s);
+/* diag.cuh:175-190 (write_csv's data): host arrays of Nx/2+1 (any may be NULL): kx = 2 pi k / Lx
+ * and the averaged E_u, E_v. Synchronises s; returns the number of samples. */
+int nse2d_spectra1d_read(nse2d_spectra1d_t sp, double* kx, double* Eu, double* Ev, hipStream_t s);
+/* diag.cuh:156-164 (destructor) */
+void nse2d_spectra1d_destroy(nse2d_spectra1d_t sp);
+
 /* ------------------------------------------------------------------------------------------
  * 4. Diagnostics reductions (diag.cuh:12-113), same names and parameter order. Outputs are
  *    the raw sums (the host applies 1/A, nu, 1/2 as main.cu:627-675 does).

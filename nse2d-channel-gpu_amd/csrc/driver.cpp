@@ -601,6 +601,12 @@ int run(Run& R) {
     double* d_v = dalloc<double>(N);
     double* d_sc = dalloc<double>(S_NSLOTS);
     double* d_mom = dalloc<double>(5 * (size_t)Ny);
+    // x-spectra machinery (main.cu:265-268)
+    nse2d_spectra1d_t spec = nullptr;
+    if (R.spectraEvery > 0) {
+        spec = nse2d_spectra1d_create(Nx, Ny, G.Lx, stream);
+        if (!spec) { std::fprintf(stderr, "ERROR: %s\n", nse2d_last_error()); return 1; }
+    }
 
     double t_start = 0.0;
     {
@@ -841,6 +847,10 @@ int run(Run& R) {
                 }
             }
         }
+        if (spec && (step % R.spectraEvery == 0)) {  // main.cu:735-744
+            launch_uv_from_psi_mapped(d_psi, d_u, d_v, Nx, Ny, inv_dx, inv_deta, d_a_node, stream);
+            nse2d_spectra1d_accumulate(spec, d_u, d_v, stream);
+        }
         if (R.profile && (step % R.timeEvery == 0)) {
             float ms = 0.f;
             NSE2D_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
@@ -860,6 +870,17 @@ int run(Run& R) {
         utau_mean = std::sqrt(0.5 * (std::fabs(-nu * sc[S_MWB]) + std::fabs(-nu * sc[S_MWT])));
     }
     write_profiles_csv(prof_path, G, P, utau_mean, T_accum, H);
+    if (spec) {  // main.cu:776-780, diag.cuh:175-190
+        const int nk = Nx / 2 + 1;
+        std::vector<double> kx(nk), eu(nk), ev(nk);
+        nse2d_spectra1d_read(spec, kx.data(), eu.data(), ev.data(), stream);
+        std::ofstream f(R.outdir + "/spectra_kx.csv");
+        f.setf(std::ios::scientific);
+        f.precision(10);
+        f << "kx,E_u,E_v,E_tot\n";
+        for (int k = 0; k < nk; ++k) f << kx[k] << "," << eu[k] << "," << ev[k] << "," << (eu[k] + ev[k]) << "\n";
+        nse2d_spectra1d_destroy(spec);
+    }
     const double el = std::chrono::duration<double>(std::chrono::steady_clock::now() - wall0).count();
     tee(prog, "Done. steps=%lld t=%.6f wall=%.3fs\n", step, t, el);
 

@@ -45,4 +45,11 @@ void poisson_inverse_group(PoissonImpl* p, void* const* d_hats, double* const* d
                            int n, const double* dy_edge, double psi_bot, double psi_top, int free_slip,
                            hipStream_t s);
 
+// Time-averaged x-spectra of u and v (diag.cuh:123-191), on a plan's row FFT.
+struct Spectra1DImpl;
+Spectra1DImpl* spectra1d_create(int Nx, int Ny, double Lx, hipStream_t s);
+void spectra1d_accumulate(Spectra1DImpl* sp, const double* d_u, const double* d_v, hipStream_t s);
+int spectra1d_read(Spectra1DImpl* sp, double* kx, double* Eu, double* Ev, hipStream_t s);
+void spectra1d_destroy(Spectra1DImpl* sp);
+
 }  // namespace nse2d

@@ -783,6 +783,77 @@ void poisson_solve(PoissonImpl* p, const double* omega, double* psi, double psi_
     for (auto& ev : e) NSE2D_HIP_CHECK(hipEventDestroy(ev));
 }
 
+// ---- x-spectra of u and v (diag.cuh:123-191, diag.cu:869-898): every accumulate transforms
+// each row of u and v (half spectrum, the plan's row FFT) and adds, per k = 0..Nx/2,
+//   E(k) += wx(k) / (Nx^2 Ny) * sum_j |X_j(k)|^2,   wx = 2 for 0 < k < Nx/2, 1 otherwise.
+namespace {
+__global__ void spectra_rows_kernel(const cplx* __restrict__ hat, int Ny, int M, double norm1d,
+                                    double* __restrict__ acc) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > M) return;
+    const double wx = (k > 0 && k < M) ? 2.0 : 1.0;
+    double sum = 0.0;
+    for (int j = 0; j < Ny; ++j) {
+        const cplx* row = hat + (size_t)j * M;
+        if (k == 0) { const double r = row[0].x; sum += r * r; }          // X_0 (real)
+        else if (k == M) { const double r = row[0].y; sum += r * r; }     // X_M (real)
+        else { const cplx z = row[k]; sum += (z.x * z.x + z.y * z.y); }
+    }
+    acc[k] += norm1d * wx * sum;
+}
+}  // namespace
+
+struct Spectra1DImpl {
+    PoissonImpl* plan = nullptr;  // only its row FFT and spectrum buffer are used
+    int Nx = 0, Ny = 0, nk = 0, samples = 0;
+    double Lx = 0, norm1d = 0;
+    double* d_acc = nullptr;      // [2][nk]: E_u, E_v sums
+};
+
+Spectra1DImpl* spectra1d_create(int Nx, int Ny, double Lx, hipStream_t s) {
+    PoissonImpl* plan = poisson_create(Nx, Ny, Lx, s);
+    if (!plan) return nullptr;
+    auto* sp = new Spectra1DImpl;
+    sp->plan = plan;
+    sp->Nx = Nx; sp->Ny = Ny; sp->Lx = Lx; sp->nk = Nx / 2 + 1;
+    sp->norm1d = 1.0 / ((double)Nx * (double)Nx * (double)Ny);
+    NSE2D_HIP_CHECK(hipMalloc(&sp->d_acc, sizeof(double) * 2 * sp->nk));
+    NSE2D_HIP_CHECK(hipMemsetAsync(sp->d_acc, 0, sizeof(double) * 2 * sp->nk, s));
+    return sp;
+}
+
+void spectra1d_accumulate(Spectra1DImpl* sp, const double* d_u, const double* d_v, hipStream_t s) {
+    const int M = sp->Nx / 2;
+    for (int c = 0; c < 2; ++c) {
+        poisson_forward(sp->plan, c == 0 ? d_u : d_v, s);
+        spectra_rows_kernel<<<ceil_div(M + 1, 256), 256, 0, s>>>(sp->plan->d_hat, sp->Ny, M, sp->norm1d,
+                                                                 sp->d_acc + (size_t)c * sp->nk);
+        NSE2D_LAUNCH_CHECK();
+    }
+    sp->samples++;
+}
+
+int spectra1d_read(Spectra1DImpl* sp, double* kx, double* Eu, double* Ev, hipStream_t s) {
+    std::vector<double> acc(2 * (size_t)sp->nk);
+    NSE2D_HIP_CHECK(hipMemcpyAsync(acc.data(), sp->d_acc, sizeof(double) * acc.size(), hipMemcpyDeviceToHost, s));
+    NSE2D_HIP_CHECK(hipStreamSynchronize(s));
+    for (int k = 0; k < sp->nk; ++k) {
+        if (kx) kx[k] = (2.0 * kPi * (double)k) / sp->Lx;  // diag.cu:870-875
+        double eu = acc[k], ev = acc[sp->nk + k];
+        if (sp->samples > 0) { eu /= sp->samples; ev /= sp->samples; }
+        if (Eu) Eu[k] = eu;
+        if (Ev) Ev[k] = ev;
+    }
+    return sp->samples;
+}
+
+void spectra1d_destroy(Spectra1DImpl* sp) {
+    if (!sp) return;
+    poisson_destroy(sp->plan);
+    (void)hipFree(sp->d_acc);
+    delete sp;
+}
+
 void poisson_destroy(PoissonImpl* p) {
     if (!p) return;
     (void)hipFree(p->d_tw); (void)hipFree(p->d_twp); (void)hipFree(p->d_kx2); (void)hipFree(p->d_cpb); (void)hipFree(p->d_gny);
@@ -794,6 +865,9 @@ void poisson_destroy(PoissonImpl* p) {
 
 struct nse2d_poisson {
     nse2d::PoissonImpl* impl;
+};
+struct nse2d_spectra1d {
+    nse2d::Spectra1DImpl* impl;
 };
 
 extern "C" {
@@ -823,6 +897,23 @@ void poisson2d_destroy(nse2d_poisson_t p) {
     if (!p) return;
     nse2d::poisson_destroy(p->impl);
     delete p;
+}
+
+nse2d_spectra1d_t nse2d_spectra1d_create(int Nx, int Ny, double Lx, hipStream_t s) {
+    nse2d::Spectra1DImpl* impl = nse2d::spectra1d_create(Nx, Ny, Lx, s);
+    if (!impl) return nullptr;
+    return new nse2d_spectra1d{impl};
+}
+void nse2d_spectra1d_accumulate(nse2d_spectra1d_t sp, const double* d_u, const double* d_v, hipStream_t s) {
+    nse2d::spectra1d_accumulate(sp->impl, d_u, d_v, s);
+}
+int nse2d_spectra1d_read(nse2d_spectra1d_t sp, double* kx, double* Eu, double* Ev, hipStream_t s) {
+    return nse2d::spectra1d_read(sp->impl, kx, Eu, Ev, s);
+}
+void nse2d_spectra1d_destroy(nse2d_spectra1d_t sp) {
+    if (!sp) return;
+    nse2d::spectra1d_destroy(sp->impl);
+    delete sp;
 }
 
 }  // extern "C"

@@ -77,6 +77,11 @@ SIGNATURES = {
     "nse2d_channel_wall_bookkeeping": (None, [_vp, _p, _d]),
     "nse2d_channel_profile": (None, [_vp, _i]),
     "nse2d_channel_profile_read": (None, [_vp, _p, ctypes.POINTER(ctypes.c_longlong)]),
+    # spectra
+    "nse2d_spectra1d_create": (_vp, [_i, _i, _d, _stream]),
+    "nse2d_spectra1d_accumulate": (None, [_vp, _vp, _vp, _stream]),
+    "nse2d_spectra1d_read": (_i, [_vp, _vp, _vp, _vp, _stream]),
+    "nse2d_spectra1d_destroy": (None, [_vp]),
     # ensembles
     "nse2d_ensemble_create": (_vp, [_vp, _i, _stream]),
     "nse2d_ensemble_destroy": (None, [_vp]),

@@ -646,6 +646,37 @@ extern "C" void orc_poisson_solve_ld(const double* omega, double* psi, int Nx, i
     }
 }
 
+// Spectra1D::accumulate (diag.cuh:165-174) with spectra1d_rowReduce_kernel (diag.cu:876-893):
+// D2Z of every row of u and v (the real rows as complex rows, k = 0..Nx/2 kept), then
+// acc[k] += norm1d * wx * sum_j |X_j(k)|^2, norm1d = 1/(Nx^2 Ny), wx = 2 for 0 < k < Nx/2.
+// acc holds E_u in [0, Nx/2] and E_v in [Nx/2+1, Nx+1].
+extern "C" void orc_spectra1d_accumulate(const double* u, const double* v, int Nx, int Ny, double* acc) {
+    typedef std::complex<double> C;
+    const int nk = Nx / 2 + 1;
+    const double norm1d = 1.0 / ((double)Nx * (double)Nx * (double)Ny);
+    FFTPlan plan(Nx);
+    for (int c = 0; c < 2; ++c) {
+        const double* f = c == 0 ? u : v;
+        std::vector<double> sums((size_t)Ny * nk);
+#pragma omp parallel
+        {
+            std::vector<C> row(Nx), scratch;
+#pragma omp for schedule(static)
+            for (int j = 0; j < Ny; ++j) {
+                for (int i = 0; i < Nx; ++i) row[i] = C(f[ID(j, i, Nx)], 0.0);
+                plan.run(row.data(), false, scratch);
+                for (int k = 0; k < nk; ++k) sums[(size_t)j * nk + k] = row[k].real() * row[k].real() + row[k].imag() * row[k].imag();
+            }
+        }
+        for (int k = 0; k < nk; ++k) {
+            const double wx = (k > 0 && k < nk - 1) ? 2.0 : 1.0;
+            double sum = 0.0;
+            for (int j = 0; j < Ny; ++j) sum += sums[(size_t)j * nk + k];
+            acc[(size_t)c * nk + k] += norm1d * wx * sum;
+        }
+    }
+}
+
 // =============================================================================================
 // diag.cu reductions
 // =============================================================================================
@@ -1660,6 +1691,14 @@ int run_main_impl(int argc, char** argv) {
     }
     std::vector<double> Hu(Ny, 0.0), Hv(Ny, 0.0), Hu2(Ny, 0.0), Hv2(Ny, 0.0), Huv(Ny, 0.0);
     std::vector<double> su(Ny), sv(Ny), su2(Ny), sv2(Ny), suv(Ny);
+    // x-spectra (main.cu:265-268): velocity fields and the accumulators
+    std::vector<double> su_f, sv_f, spec_acc;
+    long long spec_samples = 0;
+    if (R.spectraEvery > 0) {
+        su_f.assign(S.N, 0.0);
+        sv_f.assign(S.N, 0.0);
+        spec_acc.assign(2 * (size_t)(G.Nx / 2 + 1), 0.0);
+    }
     double T_accum = 0.0, last_stats_t = t_start;
     double t = t_start, dt = P.dt_init;
     long long step = 0;
@@ -1716,6 +1755,12 @@ int run_main_impl(int argc, char** argv) {
                 }
             }
         }
+        if (R.spectraEvery > 0 && (step % R.spectraEvery == 0)) {  // main.cu:735-744
+            orc_uv_from_psi_mapped(S.psi.data(), su_f.data(), sv_f.data(), G.Nx, G.Ny, S.inv_dx, S.inv_deta,
+                                   G.a_node.data());
+            orc_spectra1d_accumulate(su_f.data(), sv_f.data(), G.Nx, G.Ny, spec_acc.data());
+            ++spec_samples;
+        }
     }
     double utau_mean = 0.0;  // main.cu:757-774
     if (T_diag > 0) utau_mean = std::sqrt(tauw_abs_dt_sum / T_diag);
@@ -1725,6 +1770,19 @@ int run_main_impl(int argc, char** argv) {
         utau_mean = std::sqrt(0.5 * (std::fabs(-nu * mw[0]) + std::fabs(-nu * mw[1])));
     }
     write_profiles_csv(prof_path, G, P, utau_mean, T_accum, Hu, Hv, Hu2, Hv2, Huv);
+    if (R.spectraEvery > 0) {  // main.cu:776-780, diag.cuh:175-190
+        const int nk = G.Nx / 2 + 1;
+        std::ofstream f(R.outdir + "/spectra_kx.csv");
+        f.setf(std::ios::scientific);
+        f.precision(10);
+        f << "kx,E_u,E_v,E_tot\n";
+        for (int k = 0; k < nk; ++k) {
+            double eu = spec_acc[k], ev = spec_acc[nk + k];
+            if (spec_samples > 0) { eu /= spec_samples; ev /= spec_samples; }
+            const double kx = (2.0 * PI * (double)k) / G.Lx;  // diag.cu:870-875
+            f << kx << "," << eu << "," << ev << "," << (eu + ev) << "\n";
+        }
+    }
     return 0;
 }
 

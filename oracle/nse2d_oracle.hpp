@@ -82,6 +82,10 @@ void orc_poisson_solve_ld(const double* omega, double* psi, int Nx, int Ny, doub
                           const double* a_node, const double* a_edge, double inv_deta,
                           double psi_bot, double psi_top);
 
+// ---- diag.cuh:165-174 + diag.cu:876-893 (Spectra1D::accumulate): adds one sample of the x-spectra
+// of u and v to acc (E_u at [0, Nx/2], E_v at [Nx/2+1, Nx+1]).
+void orc_spectra1d_accumulate(const double* u, const double* v, int Nx, int Ny, double* acc);
+
 // ---- diag.cu reductions. Results are the raw sums before the host normalisation, as the
 // reference launchers leave them in their device scalars.
 void orc_enstrophy_diss_mapped(const double* w, int Nx, int Ny, double inv_dx, double inv_deta,

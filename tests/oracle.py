@@ -33,6 +33,7 @@ SIG = {
     "orc_poisson_tridiag_mapped": (None, [_vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d]),
     "orc_poisson_solve": (None, [_vp, _vp, _i, _i, _d, _vp, _vp, _d, _d, _d]),
     "orc_poisson_solve_ld": (None, [_vp, _vp, _i, _i, _d, _vp, _vp, _d, _d, _d]),
+    "orc_spectra1d_accumulate": (None, [_vp, _vp, _i, _i, _vp]),
     "orc_enstrophy_diss_mapped": (None, [_vp, _i, _i, _d, _d, _vp, _vp, _vp]),
     "orc_enstrophy_diss_mapped_full": (None, [_vp, _i, _i, _d, _d, _vp, _vp, _vp]),
     "orc_enstrophy_diss_balance_mapped": (None, [_vp, _i, _i, _d, _d, _vp, _vp, _vp, _vp]),

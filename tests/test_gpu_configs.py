@@ -180,3 +180,19 @@ def test_large_grid_16384x8193_poisson_and_stages():
     # the x-differences cancel more at dx = 2 pi / 16384, so the per-operator bar of 1e-13 applies
     for k, v in errs.items():
         assert v <= (1e-12 if k == "rhs" else 1e-13), (k, v)
+
+
+def test_spectra_csv_matches_oracle_driver(tmp_path):
+    """--spectraEvery (main.cu:735-744, 776-780): the time-averaged x-spectra file of the HIP driver
+    against the oracle driver's, on a 128x65 noisy channel (20 steps, a sample every 4)."""
+    args = ["--Nx", "128", "--Ny", "65", "--Lx", repr(2 * math.pi), "--stretch", "tanh", "--beta", "2", "--Ub", "1",
+            "--Reb", "2000", "--init", "lamband", "--amp", "0.05", "--band_kmax", "16", "--seed", "5", "--dt", "1e-3",
+            "--no-adapt", "--tend", "0.02", "--diag_save", "0.01", "--snap_save", "100", "--spectraEvery", "4"]
+    dg, do = run_both(args, str(tmp_path))
+    g = np.genfromtxt(os.path.join(dg, "spectra_kx.csv"), delimiter=",", names=True)
+    o = np.genfromtxt(os.path.join(do, "spectra_kx.csv"), delimiter=",", names=True)
+    assert g.dtype.names == ("kx", "E_u", "E_v", "E_tot") and len(g) == 65 == len(o)
+    for col in g.dtype.names:
+        e = np.max(np.abs(g[col] - o[col])) / np.max(np.abs(o[col]))
+        print(f"[spectra csv] {col}: {e:.2e}", flush=True)
+        assert e <= 1e-9, col

@@ -232,3 +232,29 @@ def test_max_uv_and_row_moments(dcase):
     abs_rows = [np.sum(np.abs(t), axis=1) for t in (u, v, u * u, v * v, u * v)]
     for o, r, a in zip(outs, refs, abs_rows):
         assert np.all(np.abs(o.download() - r) <= 1e-13 * a + 1e-300)
+
+
+@pytest.mark.parametrize("Nx,Ny", [(64, 17), (512, 33), (4096, 65)])
+def test_spectra1d_matches_oracle(Nx, Ny):
+    """nse2d_spectra1d (diag.cuh:123-191 on the row FFT) against the oracle's restatement, over
+    two accumulated samples, on the LDS and the register row-FFT paths."""
+    print(f"[spectra] {Nx}x{Ny}", flush=True)
+    rng = np.random.default_rng(Nx)
+    L = nh.lib()
+    sp = L.nse2d_spectra1d_create(Nx, Ny, 2.5, None)
+    assert sp, nh.last_error()
+    nk = Nx // 2 + 1
+    acc = np.zeros(2 * nk)
+    for _ in range(2):
+        u, v = rng.normal(size=(Ny, Nx)), rng.normal(size=(Ny, Nx))
+        du, dv = dev(u), dev(v)
+        L.nse2d_spectra1d_accumulate(sp, du.ptr, dv.ptr, None)
+        oracle.lib().orc_spectra1d_accumulate(P(u), P(v), Nx, Ny, P(acc))
+    kx, eu, ev = np.zeros(nk), np.zeros(nk), np.zeros(nk)
+    assert L.nse2d_spectra1d_read(sp, kx.ctypes.data, eu.ctypes.data, ev.ctypes.data, None) == 2
+    L.nse2d_spectra1d_destroy(sp)
+    np.testing.assert_allclose(kx, 2 * math.pi * np.arange(nk) / 2.5, rtol=1e-15)
+    for got, ref in ((eu, acc[:nk] / 2), (ev, acc[nk:] / 2)):
+        e = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+        print(f"[spectra] {Nx}x{Ny} rel err {e:.2e}", flush=True)
+        assert e <= 1e-13

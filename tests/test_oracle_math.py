@@ -222,3 +222,23 @@ def test_extended_precision_solve_matches_dense_and_fp64():
     a = oracle.poisson_solve(w, 2 * math.pi, m, 0.0, 2.0)
     b = oracle.poisson_solve(w, 2 * math.pi, m, 0.0, 2.0, extended=True)
     assert np.max(np.abs(a - b)) <= 1e-13 * np.max(np.abs(a))
+
+
+def test_spectra_parseval_and_single_mode():
+    """Spectra1D (diag.cuh:123-191): sum_k E_u(k) is the mean of u^2 over the grid points
+    (Parseval with the half-spectrum weights), and u = cos(2 pi m x / Lx) puts 1/2 at k = m."""
+    Nx, Ny = 64, 9
+    rng = np.random.default_rng(7)
+    u, v = rng.normal(size=(Ny, Nx)), rng.normal(size=(Ny, Nx))
+    acc = np.zeros(2 * (Nx // 2 + 1))
+    L().orc_spectra1d_accumulate(P(u), P(v), Nx, Ny, P(acc))
+    Eu, Ev = acc[:Nx // 2 + 1], acc[Nx // 2 + 1:]
+    assert abs(Eu.sum() - np.mean(u * u)) <= 1e-14 * np.mean(u * u)
+    assert abs(Ev.sum() - np.mean(v * v)) <= 1e-14 * np.mean(v * v)
+    m = 5
+    x = np.arange(Nx) * (3.0 / Nx)
+    c = np.ascontiguousarray(np.tile(np.cos(2 * math.pi * m * x / 3.0), (Ny, 1)))
+    acc[:] = 0
+    L().orc_spectra1d_accumulate(P(c), P(c), Nx, Ny, P(acc))
+    Eu = acc[:Nx // 2 + 1]
+    assert abs(Eu[m] - 0.5) <= 1e-14 and np.max(np.abs(np.delete(Eu, m))) <= 1e-14
