@@ -304,6 +304,33 @@ void nse2d_ensemble_sync(nse2d_ensemble_t e);
 int nse2d_ensemble_autotune(nse2d_ensemble_t e, int nsteps, char* report, size_t report_len);
 
 /* ------------------------------------------------------------------------------------------
+ * 5c. Learned subfilter closure (no reference counterpart): a CNN of six 3x3 convolutions
+ *     (4 -> 64 -> 64 -> 64 -> 64 -> 64 -> 1 channels, GELU after the first five) on images
+ *     [batch][H][W] with x periodic and zero padding beyond the walls. bf16 weights and
+ *     activations, fp32 accumulation (MFMA), fp32 master weights and Adam moments. The flat
+ *     parameter array holds, per layer, W[9][cout][cin] (tap t = 3(dy+1)+(dx+1)) then b[cout];
+ *     the first layer's cin is padded from 4 to 32 with weights that stay zero.
+ * ------------------------------------------------------------------------------------------ */
+typedef struct nse2d_cnn* nse2d_cnn_t;
+/* W a multiple of 64, H >= 3; He-initialised weights from seed. NULL on error. */
+nse2d_cnn_t nse2d_cnn_create(int H, int W, int max_batch, unsigned long long seed, hipStream_t s);
+void nse2d_cnn_destroy(nse2d_cnn_t c);
+size_t nse2d_cnn_num_params(nse2d_cnn_t c);
+/* host <-> device copies of the fp32 parameters / of the last gradients; set resets Adam */
+void nse2d_cnn_get_params(nse2d_cnn_t c, float* host);
+void nse2d_cnn_set_params(nse2d_cnn_t c, const float* host);
+void nse2d_cnn_get_grads(nse2d_cnn_t c, float* host);
+/* d_in: [B][H][W][4] fp32 (normalised inputs); d_out: [B][H][W] fp32 (may be NULL). 0 on success. */
+int nse2d_cnn_forward(nse2d_cnn_t c, const float* d_in, float* d_out, int B);
+/* Gradients of sum(d_gout * out) for the batch of the preceding forward. 0 on success. */
+int nse2d_cnn_backward(nse2d_cnn_t c, const float* d_gout, int B);
+/* forward, mean squared error over rows 1..H-2 against d_target [B][H][W], backward, Adam
+ * (beta 0.9 / 0.999, eps 1e-8) with learning rate lr; returns the loss (synchronises). */
+double nse2d_cnn_train_step(nse2d_cnn_t c, const float* d_in, const float* d_target, int B, float lr);
+/* One Adam update from the current gradients. */
+int nse2d_cnn_adam_step(nse2d_cnn_t c, float lr);
+
+/* ------------------------------------------------------------------------------------------
  * 6. Runtime helpers (device memory, streams) so that callers without another HIP binding
  *    (ctypes, cgo) can drive the library.
  * ------------------------------------------------------------------------------------------ */

@@ -1,0 +1,617 @@
+// Learned subfilter closure for the coarse channel (no reference counterpart): a CNN of six 3x3
+// convolutions (4 -> 64 -> 64 -> 64 -> 64 -> 64 -> 1 channels, GELU after the first five, about
+// 150k parameters) on [batch][H][W] images with x periodic and zero padding beyond the walls in y.
+// Activations and weights are bf16, accumulation fp32 (MFMA 32x32x16 bf16 on gfx950); the master
+// weights and the Adam moments are fp32.
+//
+// Layouts: activations NHWC [b][y][x][c] bf16 with c fastest; the 4 input channels are padded to
+// 32 (zeros). Weights of a 64-output layer: W[t][co][ci] (t = 3*(dy+1) + (dx+1)), and a transposed
+// copy WT[t][ci][co] for the input-gradient pass. The last layer (64 -> 1) is a plain fp32 dot
+// product per pixel.
+//
+// Convolution as implicit GEMM: a workgroup of 4 wavefronts owns 64 pixels of one image row and
+// all 64 output channels; wavefront w computes the 32x32 block (pixels 32(w&1).., channels
+// 32(w>>1)..) with MFMA 32x32x16: lane l supplies A[l%32][8(l/32)..+7] (8 consecutive input
+// channels of one pixel, one 16-byte load) and B[8(l/32)..+7][l%32] (8 consecutive input
+// channels of one weight row, one 16-byte load); result element (8(r/4) + 4(l/32) + r%4, l%32)
+// lands in accumulator r of lane l. The weight gradient is the same product with the pixel
+// dimension as K: tiles of 64 pixels are staged transposed ([channel][pixel]) in LDS, with the
+// three x-shifted copies of the input rows, so that each lane reads 8 consecutive pixels.
+#include "nse2d_common.hpp"
+#include "../../include/nse2d_hip.h"
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace nse2d {
+
+void set_last_error(const std::string& msg);
+
+namespace cnn {
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+constexpr int kC = 64;      // hidden channels
+constexpr int kCin0 = 32;   // padded input channels of the first layer
+constexpr int kInputs = 4;  // real input channels
+constexpr int kLayers = 6;
+
+// GELU, tanh form (Hendrycks and Gimpel): 0.5 z (1 + tanh(s (z + 0.044715 z^3))), s = sqrt(2/pi)
+__device__ inline float gelu(float z) {
+    const float s = 0.7978845608028654f;
+    return 0.5f * z * (1.0f + tanhf(s * (z + 0.044715f * z * z * z)));
+}
+__device__ inline float gelu_grad(float z) {
+    const float s = 0.7978845608028654f;
+    const float u = s * (z + 0.044715f * z * z * z);
+    const float th = tanhf(u);
+    return 0.5f * (1.0f + th) + 0.5f * z * (1.0f - th * th) * s * (1.0f + 3.0f * 0.044715f * z * z);
+}
+
+__device__ inline bf16x8 load8(const bf16* p) { return *reinterpret_cast<const bf16x8*>(p); }
+
+enum { MODE_FWD_GELU = 0, MODE_FWD_LINEAR = 1, MODE_BWD_DATA = 2 };
+
+// out[b][y][x][co] for co < 64 over the 3x3 neighbourhood of `in` (CIN channels).
+//   MODE_FWD_GELU:  z = conv + bias; zout = bf16(z); aout = bf16(gelu(z))
+//   MODE_BWD_DATA:  taps mirrored, weights WT; g = conv; aout = bf16(g * gelu'(zprev))
+template <int CIN, int MODE>
+__global__ void __launch_bounds__(256) conv_mfma_kernel(const bf16* __restrict__ in, const bf16* __restrict__ w,
+                                                        const float* __restrict__ bias, bf16* __restrict__ zout,
+                                                        bf16* __restrict__ aout, const bf16* __restrict__ zprev,
+                                                        int H, int W) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nseg = W >> 6;
+    const int seg = blockIdx.x % nseg, y = (blockIdx.x / nseg) % H, b = blockIdx.x / (nseg * H);
+    const int ph = wave & 1, ch = wave >> 1;
+    const int i = lane & 31, kh = lane >> 5;
+    const int x = (seg << 6) + (ph << 5) + i;  // this lane's A-row pixel
+    const int co = (ch << 5) + i;              // this lane's B-column channel
+    floatx16 acc = {0};
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+        int dy = t / 3 - 1, dx = t % 3 - 1;
+        if (MODE == MODE_BWD_DATA) { dy = -dy; dx = -dx; }
+        const int yy = y + dy;
+        if (yy < 0 || yy >= H) continue;  // zero padding beyond the walls (uniform over the workgroup)
+        int xx = x + dx;
+        xx = xx < 0 ? xx + W : (xx >= W ? xx - W : xx);
+        const bf16* arow = in + (((size_t)b * H + yy) * W + xx) * CIN + kh * 8;
+        const bf16* brow = w + ((size_t)t * kC + co) * CIN + kh * 8;
+#pragma unroll
+        for (int kb = 0; kb < CIN / 16; ++kb)
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(load8(arow + kb * 16), load8(brow + kb * 16), acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int px = (seg << 6) + (ph << 5) + ((r >> 2) << 3) + (kh << 2) + (r & 3);
+        const int oc = (ch << 5) + i;
+        const size_t o = (((size_t)b * H + y) * W + px) * kC + oc;
+        if (MODE == MODE_BWD_DATA) {
+            aout[o] = (bf16)(acc[r] * gelu_grad((float)zprev[o]));
+        } else {
+            const float z = acc[r] + bias[oc];
+            const bf16 zb = (bf16)z;
+            zout[o] = zb;
+            if (MODE == MODE_FWD_GELU) aout[o] = (bf16)gelu((float)zb);
+        }
+    }
+}
+
+// Partial weight gradient of a 64-output layer over a strip of row segments:
+//   dW[t][co][ci] += sum_pixels dZ[p][co] * X[p + off_t][ci]
+// Workgroup: (CIN/32)*2 wavefronts, wavefront w owns co block w/(CIN/32), ci block w%(CIN/32), all 9
+// taps (9 accumulators of 16). LDS: dZ^T [64][64 px] and the input rows y-1..y+1 transposed and
+// shifted by dx = -1, 0, 1: X^T[dx][dy][ci][64 px].
+template <int CIN>
+__global__ void __launch_bounds__(CIN * 4) conv_wgrad_kernel(const bf16* __restrict__ x, const bf16* __restrict__ dz,
+                                                             float* __restrict__ partial, int H, int W, int B,
+                                                             int segs_per_block) {
+    constexpr int NT = CIN * 4;  // threads
+    constexpr int PAD = 8;       // pixel padding of the LDS rows (bank spread)
+    constexpr int LP = 64 + PAD;
+    __shared__ alignas(16) bf16 dzT[kC][LP];
+    __shared__ alignas(16) bf16 xT[3][3][CIN][LP];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    constexpr int NCI = CIN / 32;
+    const int cb = wave / NCI, ib = wave % NCI;
+    const int i = lane & 31, kh = lane >> 5;
+    floatx16 acc[9];
+#pragma unroll
+    for (int t = 0; t < 9; ++t) acc[t] = floatx16{0};
+    const int nseg = W >> 6, total = B * H * nseg;
+    const int s0 = blockIdx.x * segs_per_block, s1 = min(total, s0 + segs_per_block);
+    for (int sg = s0; sg < s1; ++sg) {
+        const int seg = sg % nseg, y = (sg / nseg) % H, b = sg / (nseg * H);
+        const int x0 = seg << 6;
+        // stage dZ^T: 64 px x 64 ch, 8 channels per load
+        for (int e = tid; e < 64 * (kC / 8); e += NT) {
+            const int p = e / (kC / 8), c8 = (e % (kC / 8)) * 8;
+            const bf16x8 v = load8(dz + (((size_t)b * H + y) * W + x0 + p) * kC + c8);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) dzT[c8 + q][p] = v[q];
+        }
+        // stage the input rows: 66 px (x0-1 .. x0+64) of rows y-1..y+1, into the three shifted copies
+        for (int e = tid; e < 3 * 66 * (CIN / 8); e += NT) {
+            const int dyi = e / (66 * (CIN / 8)), rem = e % (66 * (CIN / 8));
+            const int q = rem / (CIN / 8), c8 = (rem % (CIN / 8)) * 8;
+            const int yy = y + dyi - 1;
+            int xx = x0 - 1 + q;
+            xx = xx < 0 ? xx + W : (xx >= W ? xx - W : xx);
+            bf16x8 v;
+            if (yy >= 0 && yy < H) v = load8(x + (((size_t)b * H + yy) * W + xx) * CIN + c8);
+            else
+#pragma unroll
+                for (int qq = 0; qq < 8; ++qq) v[qq] = (bf16)0.0f;
+#pragma unroll
+            for (int dxi = 0; dxi < 3; ++dxi) {
+                const int p = q - dxi;  // copy dx = dxi-1 holds X[x0 + p + dx] at p
+                if (p >= 0 && p < 64)
+#pragma unroll
+                    for (int qq = 0; qq < 8; ++qq) xT[dxi][dyi][c8 + qq][p] = v[qq];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < 9; ++t) {
+            const int dyi = t / 3, dxi = t % 3;
+#pragma unroll
+            for (int pb = 0; pb < 4; ++pb) {
+                const bf16x8 a = *reinterpret_cast<const bf16x8*>(&dzT[(cb << 5) + i][pb * 16 + kh * 8]);
+                const bf16x8 bb = *reinterpret_cast<const bf16x8*>(&xT[dxi][dyi][(ib << 5) + i][pb * 16 + kh * 8]);
+                acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bb, acc[t], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    float* out = partial + (size_t)blockIdx.x * 9 * kC * CIN;
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int co = (cb << 5) + ((r >> 2) << 3) + (kh << 2) + (r & 3);
+            const int ci = (ib << 5) + i;
+            out[((size_t)t * kC + co) * CIN + ci] = acc[t][r];
+        }
+}
+
+// Ordered sum of nb partial blocks of n floats: out[j] = sum_k partial[k][j].
+__global__ void sum_partials_kernel(const float* __restrict__ partial, int nb, int n, float* __restrict__ out) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    float s = 0.0f;
+    for (int k = 0; k < nb; ++k) s += partial[(size_t)k * n + j];
+    out[j] = s;
+}
+
+// Bias gradient of a 64-channel layer: per-block partial column sums over pixels.
+__global__ void __launch_bounds__(256) bias_grad_kernel(const bf16* __restrict__ dz, size_t npix, int pix_per_block,
+                                                        float* __restrict__ partial) {
+    __shared__ float sm[4][kC];
+    const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const size_t p0 = (size_t)blockIdx.x * pix_per_block;
+    const size_t p1 = p0 + pix_per_block < npix ? p0 + pix_per_block : npix;
+    float s = 0.0f;
+    for (size_t p = p0 + g; p < p1; p += 4) s += (float)dz[p * kC + c];
+    sm[g][c] = s;
+    __syncthreads();
+    if (g == 0) partial[(size_t)blockIdx.x * kC + c] = sm[0][c] + sm[1][c] + sm[2][c] + sm[3][c];
+}
+
+// Last layer (64 -> 1, linear): out[p] = b + sum_t sum_ci A[p + off_t][ci] w[t][ci], fp32.
+__global__ void last_fwd_kernel(const bf16* __restrict__ a, const float* __restrict__ w, const float* __restrict__ bias,
+                                float* __restrict__ out, int H, int W, int B) {
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= (size_t)B * H * W) return;
+    const int x = (int)(p % W), y = (int)((p / W) % H), b = (int)(p / ((size_t)W * H));
+    float s = bias[0];
+    for (int t = 0; t < 9; ++t) {
+        const int yy = y + t / 3 - 1;
+        if (yy < 0 || yy >= H) continue;
+        int xx = x + t % 3 - 1;
+        xx = xx < 0 ? xx + W : (xx >= W ? xx - W : xx);
+        const bf16* row = a + (((size_t)b * H + yy) * W + xx) * kC;
+        for (int c8 = 0; c8 < kC; c8 += 8) {
+            const bf16x8 v = load8(row + c8);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) s += (float)v[q] * w[t * kC + c8 + q];
+        }
+    }
+    out[p] = s;
+}
+
+// Input gradient of the last layer, through the GELU of the layer before:
+//   dz5[p][ci] = gelu'(z5[p][ci]) * sum_t g[p - off_t] w[t][ci]
+__global__ void last_bwd_data_kernel(const float* __restrict__ g, const float* __restrict__ w,
+                                     const bf16* __restrict__ z5, bf16* __restrict__ dz5, int H, int W, int B) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)B * H * W * kC) return;
+    const int ci = (int)(e % kC);
+    const size_t p = e / kC;
+    const int x = (int)(p % W), y = (int)((p / W) % H), b = (int)(p / ((size_t)W * H));
+    float s = 0.0f;
+    for (int t = 0; t < 9; ++t) {
+        const int yy = y - (t / 3 - 1);
+        if (yy < 0 || yy >= H) continue;
+        int xx = x - (t % 3 - 1);
+        xx = xx < 0 ? xx + W : (xx >= W ? xx - W : xx);
+        s += g[((size_t)b * H + yy) * W + xx] * w[t * kC + ci];
+    }
+    dz5[e] = (bf16)(s * gelu_grad((float)z5[e]));
+}
+
+// Weight and bias gradient partials of the last layer: one block per pixel chunk, 9*64 + 1 sums.
+__global__ void __launch_bounds__(256) last_wgrad_kernel(const float* __restrict__ g, const bf16* __restrict__ a,
+                                                         int H, int W, int B, int pix_per_block,
+                                                         float* __restrict__ partial) {
+    __shared__ float sm[4][9 * kC + 1];
+    const int tid = threadIdx.x, c = tid & 63, grp = tid >> 6;
+    const size_t np = (size_t)B * H * W;
+    const size_t p0 = (size_t)blockIdx.x * pix_per_block, p1 = p0 + pix_per_block < np ? p0 + pix_per_block : np;
+    float acc[9];
+    for (int t = 0; t < 9; ++t) acc[t] = 0.0f;
+    float gb = 0.0f;
+    for (size_t p = p0 + grp; p < p1; p += 4) {
+        const int x = (int)(p % W), y = (int)((p / W) % H), b = (int)(p / ((size_t)W * H));
+        const float gv = g[p];
+        if (c == 0) gb += gv;
+        for (int t = 0; t < 9; ++t) {
+            const int yy = y + t / 3 - 1;
+            if (yy < 0 || yy >= H) continue;
+            int xx = x + t % 3 - 1;
+            xx = xx < 0 ? xx + W : (xx >= W ? xx - W : xx);
+            acc[t] += gv * (float)a[(((size_t)b * H + yy) * W + xx) * kC + c];
+        }
+    }
+    for (int t = 0; t < 9; ++t) sm[grp][t * kC + c] = acc[t];
+    if (c == 0) sm[grp][9 * kC] = gb;
+    __syncthreads();
+    for (int j = tid; j < 9 * kC + 1; j += 256)
+        partial[(size_t)blockIdx.x * (9 * kC + 1) + j] = sm[0][j] + sm[1][j] + sm[2][j] + sm[3][j];
+}
+
+// Inputs: fp32 [b][y][x][4] (already normalised) -> bf16 NHWC padded to 32 channels.
+__global__ void pack_input_kernel(const float* __restrict__ in, bf16* __restrict__ out, size_t npix) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= npix * kCin0) return;
+    const size_t p = e / kCin0;
+    const int c = (int)(e % kCin0);
+    out[e] = (bf16)(c < kInputs ? in[p * kInputs + c] : 0.0f);
+}
+
+// Mean squared error over the rows 1..H-2 (the wall rows carry no subfilter flux) and its gradient.
+__global__ void __launch_bounds__(256) mse_kernel(const float* __restrict__ pred, const float* __restrict__ target,
+                                                  float* __restrict__ grad, int H, int W, int B,
+                                                  float* __restrict__ partial) {
+    __shared__ float sm[256];
+    const size_t np = (size_t)B * H * W;
+    const double n_eff = (double)B * (H - 2) * W;
+    float s = 0.0f;
+    for (size_t p = (size_t)blockIdx.x * 256 + threadIdx.x; p < np; p += (size_t)gridDim.x * 256) {
+        const int y = (int)((p / W) % H);
+        float d = 0.0f;
+        if (y > 0 && y < H - 1) d = pred[p] - target[p];
+        grad[p] = (float)(2.0 * d / n_eff);
+        s += d * d;
+    }
+    sm[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sm[threadIdx.x] += sm[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = sm[0];
+}
+
+// Adam (Kingma and Ba), fp32 master weights; also refreshes the bf16 copies of the conv layers.
+__global__ void adam_kernel(float* __restrict__ p, float* __restrict__ m, float* __restrict__ v,
+                            const float* __restrict__ g, size_t n, float lr, float b1, float b2, float eps,
+                            float c1, float c2) {
+    const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const float gj = g[j];
+    const float mj = b1 * m[j] + (1.0f - b1) * gj;
+    const float vj = b2 * v[j] + (1.0f - b2) * gj * gj;
+    m[j] = mj;
+    v[j] = vj;
+    p[j] -= lr * (mj / c1) / (sqrtf(vj / c2) + eps);
+}
+
+// bf16 weights of a 64-output conv layer from the fp32 master W[t][co][ci] (CIN channels), and the
+// transposed copy WT[t][ci][co] (64-channel layers).
+__global__ void weights_to_bf16_kernel(const float* __restrict__ w, int CIN, bf16* __restrict__ wb,
+                                       bf16* __restrict__ wt) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= 9 * kC * CIN) return;
+    const int ci = e % CIN, co = (e / CIN) % kC, t = e / (CIN * kC);
+    const float val = w[e];
+    wb[e] = (bf16)val;
+    if (wt) wt[((size_t)t * kC + ci) * kC + co] = (bf16)val;
+}
+
+}  // namespace cnn
+
+// ------------------------------------------------------------------------------------------------
+struct CnnImpl {
+    int H = 0, W = 0, maxB = 0;
+    hipStream_t s = nullptr;
+    // parameters, fp32, one flat array: per layer W then b. Layer l in [0,5): W[9][64][cin_l], b[64];
+    // layer 5: W[9][64], b[1].
+    std::vector<size_t> off_w, off_b, n_w;
+    size_t nparam = 0;
+    float *p = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;
+    long long step = 0;
+    cnn::bf16 *wb[5] = {nullptr}, *wt[5] = {nullptr};  // bf16 copies of the conv layers
+    cnn::bf16* x0 = nullptr;                            // packed input [B][H][W][32]
+    cnn::bf16 *z[5] = {nullptr}, *a[5] = {nullptr};     // pre-activations and activations
+    cnn::bf16 *dz0 = nullptr, *dz1 = nullptr;           // gradient ping-pong [B][H][W][64]
+    float *out = nullptr, *gout = nullptr, *partial = nullptr, *loss_part = nullptr;
+    size_t partial_floats = 0;
+    int wgrad_blocks = 0;
+};
+
+namespace {
+
+// input channels of conv layer l in the parameter layout (layer 0 keeps its 32 padded channels:
+// the padded inputs are zero, so their weights get zero gradients and stay at their zero start)
+int cinp_of(int l) { return l == 0 ? cnn::kCin0 : cnn::kC; }
+
+void refresh_bf16(CnnImpl* c) {
+    for (int l = 0; l < 5; ++l) {
+        const int CIN = cinp_of(l), n = 9 * cnn::kC * CIN;
+        cnn::weights_to_bf16_kernel<<<ceil_div(n, 256), 256, 0, c->s>>>(c->p + c->off_w[l], CIN, c->wb[l],
+                                                                        l > 0 ? c->wt[l] : nullptr);
+        NSE2D_LAUNCH_CHECK();
+    }
+}
+
+template <int CIN, int MODE>
+void launch_conv(CnnImpl* c, const cnn::bf16* in, const cnn::bf16* w, const float* bias, cnn::bf16* zo, cnn::bf16* ao,
+                 const cnn::bf16* zp, int B) {
+    const unsigned blocks = (unsigned)B * c->H * (c->W / 64);
+    cnn::conv_mfma_kernel<CIN, MODE><<<blocks, 256, 0, c->s>>>(in, w, bias, zo, ao, zp, c->H, c->W);
+    NSE2D_LAUNCH_CHECK();
+}
+
+void forward(CnnImpl* c, const float* d_in, int B) {
+    const size_t npix = (size_t)B * c->H * c->W;
+    cnn::pack_input_kernel<<<ceil_div(npix * cnn::kCin0, 256), 256, 0, c->s>>>(d_in, c->x0, npix);
+    NSE2D_LAUNCH_CHECK();
+    launch_conv<cnn::kCin0, cnn::MODE_FWD_GELU>(c, c->x0, c->wb[0], c->p + c->off_b[0], c->z[0], c->a[0], nullptr, B);
+    for (int l = 1; l < 5; ++l)
+        launch_conv<cnn::kC, cnn::MODE_FWD_GELU>(c, c->a[l - 1], c->wb[l], c->p + c->off_b[l], c->z[l], c->a[l],
+                                                 nullptr, B);
+    cnn::last_fwd_kernel<<<ceil_div(npix, 256), 256, 0, c->s>>>(c->a[4], c->p + c->off_w[5], c->p + c->off_b[5],
+                                                               c->out, c->H, c->W, B);
+    NSE2D_LAUNCH_CHECK();
+}
+
+template <int CIN>
+void wgrad(CnnImpl* c, const cnn::bf16* x, const cnn::bf16* dz, float* gw, float* gb, int B) {
+    const int total = B * c->H * (c->W / 64);
+    const int nb = std::min(c->wgrad_blocks, total);
+    const int per = (total + nb - 1) / nb;
+    const int used = (total + per - 1) / per;
+    cnn::conv_wgrad_kernel<CIN><<<used, CIN * 4, 0, c->s>>>(x, dz, c->partial, c->H, c->W, B, per);
+    NSE2D_LAUNCH_CHECK();
+    const int n = 9 * cnn::kC * CIN;
+    cnn::sum_partials_kernel<<<ceil_div(n, 256), 256, 0, c->s>>>(c->partial, used, n, gw);
+    NSE2D_LAUNCH_CHECK();
+    const size_t npix = (size_t)B * c->H * c->W;
+    const int ppb = 4096;
+    const int nbb = (int)ceil_div(npix, ppb);
+    cnn::bias_grad_kernel<<<nbb, 256, 0, c->s>>>(dz, npix, ppb, c->partial);
+    NSE2D_LAUNCH_CHECK();
+    cnn::sum_partials_kernel<<<1, 64, 0, c->s>>>(c->partial, nbb, cnn::kC, gb);
+    NSE2D_LAUNCH_CHECK();
+}
+
+// Gradients of sum(gout * out) with respect to every parameter, after forward() on the same batch.
+void backward(CnnImpl* c, const float* gout, int B) {
+    const size_t npix = (size_t)B * c->H * c->W;
+    {   // last layer
+        const int ppb = 2048, nb = (int)ceil_div(npix, ppb);
+        cnn::last_wgrad_kernel<<<nb, 256, 0, c->s>>>(gout, c->a[4], c->H, c->W, B, ppb, c->partial);
+        NSE2D_LAUNCH_CHECK();
+        cnn::sum_partials_kernel<<<ceil_div(9 * cnn::kC + 1, 256), 256, 0, c->s>>>(c->partial, nb, 9 * cnn::kC + 1,
+                                                                                   c->g + c->off_w[5]);
+        NSE2D_LAUNCH_CHECK();
+        cnn::last_bwd_data_kernel<<<ceil_div(npix * cnn::kC, 256), 256, 0, c->s>>>(gout, c->p + c->off_w[5], c->z[4],
+                                                                                  c->dz0, c->H, c->W, B);
+        NSE2D_LAUNCH_CHECK();
+    }
+    cnn::bf16* cur = c->dz0;
+    cnn::bf16* nxt = c->dz1;
+    for (int l = 4; l >= 1; --l) {
+        wgrad<cnn::kC>(c, c->a[l - 1], cur, c->g + c->off_w[l], c->g + c->off_b[l], B);
+        launch_conv<cnn::kC, cnn::MODE_BWD_DATA>(c, cur, c->wt[l], nullptr, nullptr, nxt, c->z[l - 1], B);
+        std::swap(cur, nxt);
+    }
+    wgrad<cnn::kCin0>(c, c->x0, cur, c->g + c->off_w[0], c->g + c->off_b[0], B);
+}
+
+// Host normal deviates: splitmix64 -> uniform (0,1) -> Box-Muller.
+struct HostNormal {
+    unsigned long long s;
+    bool have = false;
+    double spare = 0;
+    unsigned long long next() {
+        unsigned long long z = (s += 0x9E3779B97F4A7C15ULL);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        return z ^ (z >> 31);
+    }
+    double uniform() { return ((next() >> 11) + 0.5) * (1.0 / 9007199254740992.0); }
+    double operator()() {
+        if (have) { have = false; return spare; }
+        const double u1 = uniform(), u2 = uniform();
+        const double r = std::sqrt(-2.0 * std::log(u1)), a = 2.0 * kPi * u2;
+        spare = r * std::sin(a);
+        have = true;
+        return r * std::cos(a);
+    }
+};
+
+void cnn_destroy(CnnImpl* c) {
+    if (!c) return;
+    for (int l = 0; l < 5; ++l) { (void)hipFree(c->wb[l]); (void)hipFree(c->wt[l]); (void)hipFree(c->z[l]); (void)hipFree(c->a[l]); }
+    for (void* q : {(void*)c->p, (void*)c->g, (void*)c->m, (void*)c->v, (void*)c->x0, (void*)c->dz0, (void*)c->dz1,
+                    (void*)c->out, (void*)c->gout, (void*)c->partial, (void*)c->loss_part})
+        (void)hipFree(q);
+    delete c;
+}
+
+CnnImpl* cnn_create(int H, int W, int maxB, unsigned long long seed, hipStream_t s) {
+    if (H < 3 || W < 64 || W % 64 != 0 || maxB < 1) {
+        set_last_error("nse2d_cnn_create: needs H >= 3, W a multiple of 64, max_batch >= 1");
+        return nullptr;
+    }
+    auto* c = new CnnImpl;
+    c->H = H; c->W = W; c->maxB = maxB; c->s = s;
+    size_t o = 0;
+    for (int l = 0; l < cnn::kLayers; ++l) {
+        const size_t nw = l < 5 ? (size_t)9 * cnn::kC * cinp_of(l) : (size_t)9 * cnn::kC;
+        c->off_w.push_back(o);
+        c->n_w.push_back(nw);
+        o += nw;
+        c->off_b.push_back(o);
+        o += l < 5 ? cnn::kC : 1;
+    }
+    c->nparam = o;
+    // He initialisation over the real fan-in (padded input channels stay zero); zero biases
+    std::vector<float> h(o, 0.0f);
+    HostNormal rng{seed ? seed : 1ULL};
+    for (int l = 0; l < cnn::kLayers; ++l) {
+        const int cin = l == 0 ? cnn::kInputs : cnn::kC;
+        const double sd = std::sqrt(2.0 / (9.0 * cin)) * (l == 5 ? 0.1 : 1.0);
+        const int CIN = l < 5 ? cinp_of(l) : cnn::kC;
+        const int nco = l < 5 ? cnn::kC : 1;
+        for (int t = 0; t < 9; ++t)
+            for (int co = 0; co < nco; ++co)
+                for (int ci = 0; ci < CIN; ++ci) {
+                    const double r = rng();
+                    if (ci < cin) h[c->off_w[l] + ((size_t)t * nco + co) * CIN + ci] = (float)(sd * r);
+                }
+    }
+    for (float** q : {&c->p, &c->g, &c->m, &c->v}) {
+        NSE2D_HIP_CHECK(hipMalloc(q, o * sizeof(float)));
+        NSE2D_HIP_CHECK(hipMemsetAsync(*q, 0, o * sizeof(float), s));
+    }
+    NSE2D_HIP_CHECK(hipMemcpyAsync(c->p, h.data(), o * sizeof(float), hipMemcpyHostToDevice, s));
+    const size_t npix = (size_t)maxB * H * W;
+    for (int l = 0; l < 5; ++l) {
+        NSE2D_HIP_CHECK(hipMalloc(&c->wb[l], sizeof(cnn::bf16) * 9 * cnn::kC * cinp_of(l)));
+        if (l > 0) NSE2D_HIP_CHECK(hipMalloc(&c->wt[l], sizeof(cnn::bf16) * 9 * cnn::kC * cnn::kC));
+        NSE2D_HIP_CHECK(hipMalloc(&c->z[l], sizeof(cnn::bf16) * npix * cnn::kC));
+        NSE2D_HIP_CHECK(hipMalloc(&c->a[l], sizeof(cnn::bf16) * npix * cnn::kC));
+    }
+    NSE2D_HIP_CHECK(hipMalloc(&c->x0, sizeof(cnn::bf16) * npix * cnn::kCin0));
+    NSE2D_HIP_CHECK(hipMalloc(&c->dz0, sizeof(cnn::bf16) * npix * cnn::kC));
+    NSE2D_HIP_CHECK(hipMalloc(&c->dz1, sizeof(cnn::bf16) * npix * cnn::kC));
+    NSE2D_HIP_CHECK(hipMalloc(&c->out, sizeof(float) * npix));
+    NSE2D_HIP_CHECK(hipMalloc(&c->gout, sizeof(float) * npix));
+    c->wgrad_blocks = 1024;
+    const size_t pw = (size_t)c->wgrad_blocks * 9 * cnn::kC * cnn::kC;
+    const size_t pb = ceil_div(npix, 4096) * (size_t)cnn::kC + ceil_div(npix, 2048) * (size_t)(9 * cnn::kC + 1);
+    c->partial_floats = std::max(pw, pb);
+    NSE2D_HIP_CHECK(hipMalloc(&c->partial, sizeof(float) * c->partial_floats));
+    NSE2D_HIP_CHECK(hipMalloc(&c->loss_part, sizeof(float) * 1024));
+    refresh_bf16(c);
+    NSE2D_HIP_CHECK(hipStreamSynchronize(s));
+    return c;
+}
+
+double cnn_loss_and_grad(CnnImpl* c, const float* target, int B) {
+    cnn::mse_kernel<<<1024, 256, 0, c->s>>>(c->out, target, c->gout, c->H, c->W, B, c->loss_part);
+    NSE2D_LAUNCH_CHECK();
+    std::vector<float> lp(1024);
+    NSE2D_HIP_CHECK(hipMemcpyAsync(lp.data(), c->loss_part, sizeof(float) * 1024, hipMemcpyDeviceToHost, c->s));
+    NSE2D_HIP_CHECK(hipStreamSynchronize(c->s));
+    double sum = 0.0;
+    for (float v : lp) sum += v;
+    return sum / ((double)B * (c->H - 2) * c->W);
+}
+
+void cnn_adam(CnnImpl* c, float lr) {
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    c->step++;
+    const float c1 = (float)(1.0 - std::pow((double)b1, (double)c->step));
+    const float c2 = (float)(1.0 - std::pow((double)b2, (double)c->step));
+    cnn::adam_kernel<<<ceil_div(c->nparam, 256), 256, 0, c->s>>>(c->p, c->m, c->v, c->g, c->nparam, lr, b1, b2, eps,
+                                                                c1, c2);
+    NSE2D_LAUNCH_CHECK();
+    refresh_bf16(c);
+}
+
+}  // namespace
+}  // namespace nse2d
+
+struct nse2d_cnn {
+    nse2d::CnnImpl* impl;
+};
+
+using namespace nse2d;
+
+extern "C" {
+
+nse2d_cnn_t nse2d_cnn_create(int H, int W, int max_batch, unsigned long long seed, hipStream_t s) {
+    CnnImpl* c = cnn_create(H, W, max_batch, seed, s);
+    return c ? new nse2d_cnn{c} : nullptr;
+}
+void nse2d_cnn_destroy(nse2d_cnn_t c) {
+    if (!c) return;
+    cnn_destroy(c->impl);
+    delete c;
+}
+size_t nse2d_cnn_num_params(nse2d_cnn_t c) { return c->impl->nparam; }
+void nse2d_cnn_get_params(nse2d_cnn_t c, float* host) {
+    NSE2D_HIP_CHECK(hipMemcpyAsync(host, c->impl->p, sizeof(float) * c->impl->nparam, hipMemcpyDeviceToHost, c->impl->s));
+    NSE2D_HIP_CHECK(hipStreamSynchronize(c->impl->s));
+}
+void nse2d_cnn_set_params(nse2d_cnn_t c, const float* host) {
+    CnnImpl* k = c->impl;
+    NSE2D_HIP_CHECK(hipMemcpyAsync(k->p, host, sizeof(float) * k->nparam, hipMemcpyHostToDevice, k->s));
+    NSE2D_HIP_CHECK(hipMemsetAsync(k->m, 0, sizeof(float) * k->nparam, k->s));
+    NSE2D_HIP_CHECK(hipMemsetAsync(k->v, 0, sizeof(float) * k->nparam, k->s));
+    k->step = 0;
+    refresh_bf16(k);
+    NSE2D_HIP_CHECK(hipStreamSynchronize(k->s));
+}
+void nse2d_cnn_get_grads(nse2d_cnn_t c, float* host) {
+    NSE2D_HIP_CHECK(hipMemcpyAsync(host, c->impl->g, sizeof(float) * c->impl->nparam, hipMemcpyDeviceToHost, c->impl->s));
+    NSE2D_HIP_CHECK(hipStreamSynchronize(c->impl->s));
+}
+int nse2d_cnn_forward(nse2d_cnn_t c, const float* d_in, float* d_out, int B) {
+    CnnImpl* k = c->impl;
+    if (B < 1 || B > k->maxB) { set_last_error("nse2d_cnn_forward: batch outside [1, max_batch]"); return -1; }
+    forward(k, d_in, B);
+    if (d_out)
+        NSE2D_HIP_CHECK(hipMemcpyAsync(d_out, k->out, sizeof(float) * (size_t)B * k->H * k->W, hipMemcpyDeviceToDevice,
+                                       k->s));
+    return 0;
+}
+int nse2d_cnn_backward(nse2d_cnn_t c, const float* d_gout, int B) {
+    CnnImpl* k = c->impl;
+    if (B < 1 || B > k->maxB) { set_last_error("nse2d_cnn_backward: batch outside [1, max_batch]"); return -1; }
+    backward(k, d_gout, B);
+    return 0;
+}
+double nse2d_cnn_train_step(nse2d_cnn_t c, const float* d_in, const float* d_target, int B, float lr) {
+    CnnImpl* k = c->impl;
+    if (B < 1 || B > k->maxB) { set_last_error("nse2d_cnn_train_step: batch outside [1, max_batch]"); return -1.0; }
+    forward(k, d_in, B);
+    const double loss = cnn_loss_and_grad(k, d_target, B);
+    backward(k, k->gout, B);
+    cnn_adam(k, lr);
+    return loss;
+}
+int nse2d_cnn_adam_step(nse2d_cnn_t c, float lr) {
+    cnn_adam(c->impl, lr);
+    return 0;
+}
+
+}  // extern "C"

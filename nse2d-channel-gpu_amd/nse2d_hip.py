@@ -82,6 +82,17 @@ SIGNATURES = {
     "nse2d_spectra1d_accumulate": (None, [_vp, _vp, _vp, _stream]),
     "nse2d_spectra1d_read": (_i, [_vp, _vp, _vp, _vp, _stream]),
     "nse2d_spectra1d_destroy": (None, [_vp]),
+    # learned closure network
+    "nse2d_cnn_create": (_vp, [_i, _i, _i, ctypes.c_ulonglong, _stream]),
+    "nse2d_cnn_destroy": (None, [_vp]),
+    "nse2d_cnn_num_params": (_sz, [_vp]),
+    "nse2d_cnn_get_params": (None, [_vp, _vp]),
+    "nse2d_cnn_set_params": (None, [_vp, _vp]),
+    "nse2d_cnn_get_grads": (None, [_vp, _vp]),
+    "nse2d_cnn_forward": (_i, [_vp, _vp, _vp, _i]),
+    "nse2d_cnn_backward": (_i, [_vp, _vp, _i]),
+    "nse2d_cnn_train_step": (_d, [_vp, _vp, _vp, _i, ctypes.c_float]),
+    "nse2d_cnn_adam_step": (_i, [_vp, ctypes.c_float]),
     # ensembles
     "nse2d_ensemble_create": (_vp, [_vp, _i, _stream]),
     "nse2d_ensemble_destroy": (None, [_vp]),
@@ -180,6 +191,38 @@ class DeviceBuffer:
         out = np.empty(self.n, dtype=np.float64)
         lib().nse2d_memcpy_d2h(out.ctypes.data, self.ptr, out.nbytes)
         return out.reshape(shape) if shape is not None else out
+
+    def free(self):
+        if self.ptr:
+            lib().nse2d_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class DeviceArray:
+    """A device copy of a numpy array of any dtype (nse2d_dev_alloc), with download()."""
+
+    def __init__(self, a=None, shape=None, dtype=np.float32):
+        if a is not None:
+            a = np.ascontiguousarray(a)
+            shape, dtype = a.shape, a.dtype
+        self.shape, self.dtype = tuple(shape), np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.ptr = lib().nse2d_dev_alloc(max(1, self.nbytes))
+        if self.ptr is None:
+            raise MemoryError("nse2d_dev_alloc failed")
+        if a is not None:
+            lib().nse2d_memcpy_h2d(self.ptr, a.ctypes.data, self.nbytes)
+
+    def download(self):
+        out = np.empty(self.shape, self.dtype)
+        lib().nse2d_memcpy_d2h(out.ctypes.data, self.ptr, self.nbytes)
+        return out
 
     def free(self):
         if self.ptr:

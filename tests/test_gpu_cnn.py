@@ -1,0 +1,102 @@
+"""The closure network's HIP kernels (csrc/closure.hip: MFMA bf16 convolutions, weight gradients,
+the last layer, MSE, Adam) against the CPU oracle (oracle/cnn_oracle.py). With the oracle rounding
+to bf16 where the kernels store bf16, the remaining difference is fp32 against float64
+accumulation, which can move an occasional value by one bf16 step: outputs and gradients are
+compared normwise at 5e-3 / 2e-2. Against the pure float64 network the difference is the bf16
+error itself (a few 1e-3)."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import nse2d_hip as nh
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import cnn_oracle as co  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def nrel(a, b):
+    return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / np.linalg.norm(np.ravel(b)))
+
+
+def make(H, W, B, seed=5):
+    c = nh.lib().nse2d_cnn_create(H, W, B, seed, None)
+    assert c, nh.last_error()
+    n = nh.lib().nse2d_cnn_num_params(c)
+    p = np.zeros(n, np.float32)
+    nh.lib().nse2d_cnn_get_params(c, p.ctypes.data)
+    return c, p
+
+
+@pytest.mark.parametrize("H,W,B", [(9, 64, 2), (7, 128, 1)])
+def test_forward_and_gradients_match_oracle(H, W, B):
+    print(f"[cnn] {B}x{H}x{W}", flush=True)
+    c, p = make(H, W, B)
+    lay, n = co.layout()
+    assert p.size == n
+    rng = np.random.default_rng(H * W)
+    inp = rng.normal(size=(B, H, W, co.NIN)).astype(np.float32)
+    d_in, d_out = nh.DeviceArray(inp), nh.DeviceArray(shape=(B, H, W))
+    assert nh.lib().nse2d_cnn_forward(c, d_in.ptr, d_out.ptr, B) == 0
+    out = d_out.download().astype(np.float64)
+    ref, cache = co.forward(p.astype(np.float64), inp.astype(np.float64), emulate=True)
+    ref64, cache64 = co.forward(p.astype(np.float64), inp.astype(np.float64), emulate=False)
+    e, e64 = nrel(out, ref), nrel(out, ref64)
+    print(f"[cnn] forward: vs bf16 oracle {e:.2e}, vs float64 {e64:.2e}", flush=True)
+    assert e <= 5e-3 and e64 <= 5e-2
+    gout = rng.normal(size=(B, H, W)).astype(np.float32)
+    d_g = nh.DeviceArray(gout)
+    assert nh.lib().nse2d_cnn_backward(c, d_g.ptr, B) == 0
+    g = np.zeros(n, np.float32)
+    nh.lib().nse2d_cnn_get_grads(c, g.ctypes.data)
+    gr = co.backward(p.astype(np.float64), cache, gout.astype(np.float64), emulate=True)
+    gr64 = co.backward(p.astype(np.float64), cache64, gout.astype(np.float64), emulate=False)
+    for l, (ow, cout, cin, ob) in enumerate(lay):
+        sl = slice(ow, ow + 9 * cout * cin)
+        if l == 0:  # the padded input channels carry zero gradient
+            gw = g[sl].reshape(9, cout, cin)
+            assert np.all(gw[..., co.NIN:] == 0)
+        ew, eb = nrel(g[sl], gr[sl]), nrel(g[ob:ob + cout], gr[ob:ob + cout])
+        ew64 = nrel(g[sl], gr64[sl])
+        print(f"[cnn] layer {l}: dW {ew:.2e} (float64 {ew64:.2e}) db {eb:.2e}", flush=True)
+        assert ew <= 2e-2 and eb <= 2e-2 and ew64 <= 1e-1
+    nh.lib().nse2d_cnn_destroy(c)
+
+
+def test_adam_step_matches_oracle():
+    c, p = make(5, 64, 1)
+    rng = np.random.default_rng(9)
+    inp = rng.normal(size=(1, 5, 64, co.NIN)).astype(np.float32)
+    d_in = nh.DeviceArray(inp)
+    nh.lib().nse2d_cnn_forward(c, d_in.ptr, None, 1)
+    d_g = nh.DeviceArray(rng.normal(size=(1, 5, 64)).astype(np.float32))
+    nh.lib().nse2d_cnn_backward(c, d_g.ptr, 1)
+    g = np.zeros(p.size, np.float32)
+    nh.lib().nse2d_cnn_get_grads(c, g.ctypes.data)
+    m = v = np.zeros(p.size)
+    q = p.astype(np.float64)
+    for step in (1, 2):
+        assert nh.lib().nse2d_cnn_adam_step(c, 1e-3) == 0
+        q, m, v = co.adam(q, m, v, g.astype(np.float64), step, 1e-3)
+    got = np.zeros(p.size, np.float32)
+    nh.lib().nse2d_cnn_get_params(c, got.ctypes.data)
+    assert np.max(np.abs(got - q)) <= 1e-6 * max(1.0, np.max(np.abs(q)))
+    nh.lib().nse2d_cnn_destroy(c)
+
+
+def test_training_reduces_loss_on_a_fixed_batch():
+    H, W, B = 17, 64, 4
+    c, p = make(H, W, B, seed=11)
+    rng = np.random.default_rng(12)
+    inp = rng.normal(size=(B, H, W, co.NIN)).astype(np.float32)
+    tgt = (0.5 * inp[..., 0] - 0.3 * np.roll(inp[..., 1], 1, axis=2)).astype(np.float32)
+    d_in, d_t = nh.DeviceArray(inp), nh.DeviceArray(tgt)
+    losses = [nh.lib().nse2d_cnn_train_step(c, d_in.ptr, d_t.ptr, B, 1e-3) for _ in range(60)]
+    print(f"[cnn] loss {losses[0]:.3e} -> {losses[-1]:.3e}", flush=True)
+    out, _ = co.forward(p.astype(np.float64), inp.astype(np.float64), emulate=True)
+    assert abs(losses[0] - co.mse(out, tgt)[0]) <= 1e-2 * losses[0]
+    assert losses[-1] < 0.2 * losses[0]
+    nh.lib().nse2d_cnn_destroy(c)
