@@ -109,7 +109,7 @@ __global__ void __launch_bounds__(256) stage_kernel(StageArgs a) {
 // of PF raw rows in registers, and of omega^n rows in stages 2, 3), so that every load has PF
 // rows of arithmetic to arrive and each wavefront keeps PF rows of requests in flight.
 template <int STAGE, int RY, int SYW, int PF>
-__device__ inline void stage_wave_body(const StageArgs& a, int tile) {
+__device__ __forceinline__ void stage_wave_body(const StageArgs& a, int tile) {
     const int Nx = a.Nx, Ny = a.Ny;
     const int lane = threadIdx.x;
     const int ntx = Nx >> 6;
@@ -238,7 +238,7 @@ __global__ void __launch_bounds__(1024) thom_bookkeeping_kernel(double* __restri
 struct DtRule {
     double dx, deta, cfl, cvisc, nu, inv_dx2, inv_dymin2, dt_max;
 };
-__device__ inline double dt_rule(const double* m, const DtRule& r) {
+__device__ __forceinline__ double dt_rule(const double* m, const DtRule& r) {
     const double umax = m[0], vmaxm = m[2];
     const double dt_x = (umax > 0 ? r.cfl * (r.dx / umax) : 1e30);
     const double dt_y = (vmaxm > 0 ? r.cfl * (r.deta / vmaxm) : 1e30);

@@ -25,12 +25,12 @@ namespace nse2d {
 namespace fft {
 
 struct cplx { double x, y; };
-__device__ inline cplx cadd(cplx a, cplx b) { return {a.x + b.x, a.y + b.y}; }
-__device__ inline cplx csub(cplx a, cplx b) { return {a.x - b.x, a.y - b.y}; }
-__device__ inline cplx cmul(cplx a, cplx b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
-__device__ inline cplx cmulc(cplx a, cplx b) { return {a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y}; }  // a conj(b)
-__device__ inline cplx cconj(cplx a) { return {a.x, -a.y}; }
-__device__ inline cplx cscale(cplx a, double s) { return {a.x * s, a.y * s}; }
+__device__ __forceinline__ cplx cadd(cplx a, cplx b) { return {a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ cplx csub(cplx a, cplx b) { return {a.x - b.x, a.y - b.y}; }
+__device__ __forceinline__ cplx cmul(cplx a, cplx b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+__device__ __forceinline__ cplx cmulc(cplx a, cplx b) { return {a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y}; }  // a conj(b)
+__device__ __forceinline__ cplx cconj(cplx a) { return {a.x, -a.y}; }
+__device__ __forceinline__ cplx cscale(cplx a, double s) { return {a.x * s, a.y * s}; }
 
 constexpr int ilog2c(int n) { return n <= 1 ? 0 : 1 + ilog2c(n >> 1); }
 
@@ -57,16 +57,16 @@ struct Plan {
     static constexpr int TWSIZE = twoff(NPASS);
 };
 
-__device__ inline int lds_pad(int p) { return p + (p >> 4); }
+__device__ __forceinline__ int lds_pad(int p) { return p + (p >> 4); }
 
 // multiply by -i (forward) or +i (inverse)
 template <bool INV>
-__device__ inline cplx mul_mi(cplx a) { return INV ? cplx{-a.y, a.x} : cplx{a.y, -a.x}; }
+__device__ __forceinline__ cplx mul_mi(cplx a) { return INV ? cplx{-a.y, a.x} : cplx{a.y, -a.x}; }
 
 // a * exp(-+2 pi i e / N) for N in {8, 16}; e is a constant after unrolling, so the table
 // reads fold into immediates and the exact cases (multiples of pi/2) skip the multiply.
 template <int N, bool INV>
-__device__ inline cplx twiddle_const(cplx a, int e) {
+__device__ __forceinline__ cplx twiddle_const(cplx a, int e) {
     // cos and sin of m pi / 8, m = 0..15
     constexpr double C1 = 0.92387953251128675613, S1 = 0.38268343236508977173, R2 = 0.70710678118654752440;
     constexpr double cs[16] = {1, C1, R2, S1, 0, -S1, -R2, -C1, -1, -C1, -R2, -S1, 0, S1, R2, C1};
@@ -85,7 +85,7 @@ __device__ inline cplx twiddle_const(cplx a, int e) {
 // out. N = N1 * 4: DFT-4 over n2 of x[n1 + N1 n2], twiddle W_N^{n1 k2}, DFT-N1 over n1, and the
 // result X[4 k1 + k2] (found at n1 = k1, slot k2) is moved to natural order.
 template <int N, bool INV, int S = 1>
-__device__ inline void dft(cplx* x) {
+__device__ __forceinline__ void dft(cplx* x) {
     if constexpr (N == 2) {
         const cplx a = x[0], b = x[S];
         x[0] = cadd(a, b);
@@ -120,7 +120,7 @@ __device__ inline void dft(cplx* x) {
 // Pass p >= 1 (or p = 0 of the inverse) from LDS; writes LDS, or, for the inverse's last pass,
 // leaves the scaled outputs in v: v[i Rp + q] is output element tid + i T + q Ns (store_row).
 template <int M, int R, bool INV, int p>
-__device__ inline void lds_pass(cplx (&v)[R], cplx* lds, const cplx* __restrict__ twp, double scale) {
+__device__ __forceinline__ void lds_pass(cplx (&v)[R], cplx* lds, const cplx* __restrict__ twp, double scale) {
     using P = Plan<M, R>;
     constexpr int T = P::T, Rp = P::radix(p), Ns = P::ns(p), NI = R / Rp, MR = M / Rp;
     constexpr bool LAST = (p == P::NPASS - 1);
@@ -153,7 +153,7 @@ __device__ inline void lds_pass(cplx (&v)[R], cplx* lds, const cplx* __restrict_
 }
 
 template <int M, int R, bool INV, int... Ps>
-__device__ inline void lds_passes(cplx (&v)[R], cplx* lds, const cplx* twp, double scale,
+__device__ __forceinline__ void lds_passes(cplx (&v)[R], cplx* lds, const cplx* twp, double scale,
                                   std::integer_sequence<int, Ps...>) {
     (lds_pass<M, R, INV, Ps + (INV ? 0 : 1)>(v, lds, twp, scale), ...);
 }
@@ -169,7 +169,7 @@ struct LastPass {
 
 // One forward row: Nx = 2M reals -> packed half spectrum (M complex). Uses all of lds.
 template <int M, int R>
-__device__ inline void forward_row(const double* __restrict__ src_row, cplx* __restrict__ dst,
+__device__ __forceinline__ void forward_row(const double* __restrict__ src_row, cplx* __restrict__ dst,
                                    cplx* lds, const cplx* __restrict__ twp, const cplx* __restrict__ tw) {
     using P = Plan<M, R>;
     constexpr int T = P::T;
@@ -223,7 +223,7 @@ __global__ void __launch_bounds__(M / R) rows_forward_group(const double* const*
 // One inverse row: packed half spectrum -> the row's M complex values (x_2n, x_2n+1), scaled by
 // `scale`, left in v (element LastPass::pos(i, q) in v[i Rp + q]). Uses all of lds.
 template <int M, int R>
-__device__ inline void inverse_row(const cplx* __restrict__ src, cplx (&v)[R], cplx* lds,
+__device__ __forceinline__ void inverse_row(const cplx* __restrict__ src, cplx (&v)[R], cplx* lds,
                                    const cplx* __restrict__ twp, const cplx* __restrict__ tw, double scale) {
     using P = Plan<M, R>;
     constexpr int T = P::T;
@@ -260,7 +260,7 @@ struct WallClosure {
 // Ny-1, so that the two closure workgroups (rows 1 and Ny-2, each transforming two rows) are
 // dispatched first.
 template <int M, int R>
-__device__ inline void inverse_block(const cplx* __restrict__ hat, double* __restrict__ psi, cplx* lds,
+__device__ __forceinline__ void inverse_block(const cplx* __restrict__ hat, double* __restrict__ psi, cplx* lds,
                                      const cplx* __restrict__ twp, const cplx* __restrict__ tw, double scale,
                                      const WallClosure& wc, int b, int nrow) {
     using L = LastPass<M, R>;
@@ -284,7 +284,13 @@ __device__ inline void inverse_block(const cplx* __restrict__ hat, double* __res
     }
     __syncthreads();  // lds is reused
     cplx v2[R];
-    inverse_row<M, R>(hat + (size_t)(bottom ? 2 : wc.Ny - 3) * M, v2, lds, twp, tw, scale);
+    // Twiddle pointers made opaque to the compiler for the second transform: otherwise it keeps
+    // the first transform's twiddle loads live in registers for reuse here (about 64 more VGPRs
+    // in every workgroup, one wave per SIMD fewer), although only two workgroups take this path.
+    const cplx* twp2 = twp;
+    const cplx* tw2 = tw;
+    asm volatile("" : "+s"(twp2), "+s"(tw2));
+    inverse_row<M, R>(hat + (size_t)(bottom ? 2 : wc.Ny - 3) * M, v2, lds, twp2, tw2, scale);
     const double dy1 = wc.dy_edge[bottom ? 0 : wc.Ny - 2], dy2 = wc.dy_edge[bottom ? 1 : wc.Ny - 3];
     const double A = 2.0 * (1.0 / (dy1 * dy1) + 1.0 / (dy1 * dy2));
     const double B = (-2.0) * dy1 / (dy2 * (dy1 + dy2) * (dy1 + dy2));

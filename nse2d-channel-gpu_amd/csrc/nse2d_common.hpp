@@ -36,18 +36,18 @@ inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b
 // or cache lines then meet in the same L2 instead of being fetched from HBM once per XCD.
 // The map is a bijection of [0, nb) for any nb.
 constexpr int kXcds = 8;
-__device__ inline int xcd_tile(int b, int nb) {
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
     const int x = b % kXcds, q = nb / kXcds, r = nb % kXcds;
     return x * q + min(x, r) + b / kXcds;
 }
 
 // Sum over the 64 lanes of a wavefront; every lane gets the total.
-__device__ inline double wave_sum(double v) {
+__device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
 }
-__device__ inline double wave_max(double v) {
+__device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
     return v;
@@ -56,7 +56,7 @@ __device__ inline double wave_max(double v) {
 // Whole-wavefront lane shifts by one (GFX9 DPP wave_shr:1 / wave_shl:1, all 64 lanes active):
 // wave_shr1 gives lane i the value of lane i-1 and lane 0 its own `edge`; wave_shl1 gives lane i
 // the value of lane i+1 and lane 63 its own `edge`.
-__device__ inline double dpp_shift_(double v, double edge, int ctrl_is_shr) {
+__device__ __forceinline__ double dpp_shift_(double v, double edge, int ctrl_is_shr) {
     const unsigned long long vb = __double_as_longlong(v), eb = __double_as_longlong(edge);
     int lo, hi;
     if (ctrl_is_shr) {
@@ -68,13 +68,13 @@ __device__ inline double dpp_shift_(double v, double edge, int ctrl_is_shr) {
     }
     return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
 }
-__device__ inline double wave_shr1(double v, double edge) { return dpp_shift_(v, edge, 1); }
-__device__ inline double wave_shl1(double v, double edge) { return dpp_shift_(v, edge, 0); }
+__device__ __forceinline__ double wave_shr1(double v, double edge) { return dpp_shift_(v, edge, 1); }
+__device__ __forceinline__ double wave_shl1(double v, double edge) { return dpp_shift_(v, edge, 0); }
 
 // Block-wide sum of NACC accumulators; result valid in thread 0. blockDim.x must be a multiple
 // of 64 and at most 1024. `sm` needs NACC*16 doubles.
 template <int NACC>
-__device__ inline void block_sum(double (&v)[NACC], double* sm) {
+__device__ __forceinline__ void block_sum(double (&v)[NACC], double* sm) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
 #pragma unroll
     for (int a = 0; a < NACC; ++a) v[a] = wave_sum(v[a]);
@@ -93,7 +93,7 @@ __device__ inline void block_sum(double (&v)[NACC], double* sm) {
 }
 
 template <int NACC>
-__device__ inline void block_max(double (&v)[NACC], double* sm) {
+__device__ __forceinline__ void block_max(double (&v)[NACC], double* sm) {
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
 #pragma unroll
     for (int a = 0; a < NACC; ++a) v[a] = wave_max(v[a]);

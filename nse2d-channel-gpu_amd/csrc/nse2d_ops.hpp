@@ -11,7 +11,7 @@ namespace nse2d {
 // The 3x3 neighbourhood of (ix, iy) with periodic x; rows iy-1..iy+1 must exist.
 struct Stencil9 {
     double c, xp, xm, yp, ym, pp, pm, mp, mm;  // pp=(x+1,y+1) pm=(x+1,y-1) mp=(x-1,y+1) mm=(x-1,y-1)
-    __device__ static inline Stencil9 load(const double* __restrict__ f, int Nx, int ix, int iy) {
+    __device__ static __forceinline__ Stencil9 load(const double* __restrict__ f, int Nx, int ix, int iy) {
         const int xm_ = wrap_x(ix - 1, Nx), xp_ = wrap_x(ix + 1, Nx);
         const double* r0 = f + (size_t)(iy - 1) * Nx;
         const double* r1 = f + (size_t)iy * Nx;
@@ -25,7 +25,7 @@ struct Stencil9 {
 };
 
 // ops.cu:305-322: the three Arakawa terms with the row metric a on every eta difference.
-__device__ inline void arakawa_terms(const Stencil9& P, const Stencil9& W, double a,
+__device__ __forceinline__ void arakawa_terms(const Stencil9& P, const Stencil9& W, double a,
                                      double inv4dxde, double& J1, double& J2, double& J3) {
     J1 = ((P.xp - P.xm) * (a * (W.yp - W.ym)) - (a * (P.yp - P.ym)) * (W.xp - W.xm)) * inv4dxde;
     J2 = (P.xp * (a * (W.pp - W.pm)) - P.xm * (a * (W.mp - W.mm)) -
@@ -35,7 +35,7 @@ __device__ inline void arakawa_terms(const Stencil9& P, const Stencil9& W, doubl
 }
 
 // ops.cu:488-498: delta_xx + a_j [a_{j+1/2}(q_{j+1}-q_j) - a_{j-1/2}(q_j-q_{j-1})] / deta^2
-__device__ inline double mapped_laplacian(double qxm, double qc, double qxp, double qym,
+__device__ __forceinline__ double mapped_laplacian(double qxm, double qc, double qxp, double qym,
                                           double qyp, double a, double ap, double am,
                                           double inv_dx2, double inv_deta2) {
     const double dxx = (qxp - 2 * qc + qxm) * inv_dx2;
@@ -68,7 +68,7 @@ inline MmsParams make_mms_params(int Nx, double Lx, double h, double t, double n
     return m;
 }
 // ops.cu:583-624: f_omega(x_i, y) for psi_ex = A(t) sin(kx) sin^2(theta)
-__device__ inline double mms_f_omega(const MmsParams& m, int i, double y) {
+__device__ __forceinline__ double mms_f_omega(const MmsParams& m, int i, double y) {
     const double x = (double)i * m.dx;
     const double theta = m.c * (y + m.h);
     const double two_th = 2.0 * theta;
@@ -90,7 +90,7 @@ __device__ inline double mms_f_omega(const MmsParams& m, int i, double y) {
 
 // ops.cu:113-151: wall vorticity at column i from psi rows 1,2 and Ny-2,Ny-3.
 // A = 2(1/h1^2 + 1/(h1 h2)), B = -2 h1 / (h2 (h1+h2)^2); omega_wall = -(A dpsi1 + B dpsi2).
-__device__ inline void thom_values(const double* __restrict__ psi, int Nx, int Ny,
+__device__ __forceinline__ void thom_values(const double* __restrict__ psi, int Nx, int Ny,
                                    const double* __restrict__ dy_edge, double psi_bot,
                                    double psi_top, int free_slip, int i, double& wb, double& wt) {
     if (free_slip) { wb = 0.0; wt = 0.0; return; }

@@ -231,7 +231,7 @@ struct ThomasShared {
 
 // The solve of column group `grp` (columns grp*KC .. grp*KC+KC-1) of one spectrum.
 template <int RC, int KC, int T>
-__device__ inline void thomas_group(ThomasShared<RC, KC, T>& shm, int grp, cplx* __restrict__ hat, int M, int Ni,
+__device__ __forceinline__ void thomas_group(ThomasShared<RC, KC, T>& shm, int grp, cplx* __restrict__ hat, int M, int Ni,
                                     const double* __restrict__ a_sub, const double* __restrict__ c_sup,
                                     const double* __restrict__ kx2, const double* __restrict__ cpb,
                                     const double* __restrict__ gny, double bc_bot, double bc_top) {
