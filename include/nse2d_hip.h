@@ -269,6 +269,14 @@ enum { NSE2D_PROF_STAGE = 0, NSE2D_PROF_FFT_FWD = 1, NSE2D_PROF_THOMAS = 2, NSE2
 void nse2d_channel_profile(nse2d_channel_t ch, int class_mask);
 /* Synchronises the stream and returns the accumulated milliseconds and launch counts per class. */
 void nse2d_channel_profile_read(nse2d_channel_t ch, double* ms, long long* count);
+/* Optional additive RHS term (no reference counterpart; used by the learned closure of a coarse
+ * channel, section 5d): before each SSPRK stage the channel calls
+ * fn(ctx, d_omega_s, d_psi_s, d_out, s), which must fill d_out ([Ny][Nx] fp32, owned by the
+ * channel) on stream s from the stage state; the stage adds d_out to the RHS of rows 1..Ny-2.
+ * Stages with a closure run the generic stage kernel. fn = NULL detaches it. Returns 0. */
+typedef void (*nse2d_closure_fn)(void* ctx, const double* d_omega_s, const double* d_psi_s,
+                                 float* d_out, hipStream_t s);
+int nse2d_channel_set_closure(nse2d_channel_t ch, nse2d_closure_fn fn, void* ctx);
 
 /* ------------------------------------------------------------------------------------------
  * 5b. Ensembles of independent channels (no reference counterpart: the reference advances one
@@ -329,6 +337,55 @@ int nse2d_cnn_backward(nse2d_cnn_t c, const float* d_gout, int B);
 double nse2d_cnn_train_step(nse2d_cnn_t c, const float* d_in, const float* d_target, int B, float lr);
 /* One Adam update from the current gradients. */
 int nse2d_cnn_adam_step(nse2d_cnn_t c, float lr);
+/* forward and the mean squared error of nse2d_cnn_train_step, without an update (synchronises). */
+double nse2d_cnn_eval_loss(nse2d_cnn_t c, const float* d_in, const float* d_target, int B);
+
+/* ------------------------------------------------------------------------------------------
+ * 5d. Learned closure of a coarse channel (LES) trained on filtered DNS (no reference
+ *     counterpart). Filter F, fine -> coarse: box filter over ry fine cells in mapped y
+ *     (trapezoid rule, wall rows kept), then a sharp spectral cut |kx| < Nx_c/2 sampled at every
+ *     rx-th fine node. Closure target of a fine state: F(R_f) - R_c(omega_c, psi_c) on the
+ *     interior rows, where omega_c = F(omega_f) with Thom wall rows from psi_c = solve(omega_c)
+ *     and R = -J + nu lap + f(y) - r omega (ops.cu:515-544). The CNN (section 5c) maps the
+ *     normalised coarse (omega, psi, u, v) to the normalised target. FP64 everywhere but the CNN.
+ *     Every call runs on the stream given at creation; a coarse channel that uses the closure
+ *     must have been created on the same stream.
+ * ------------------------------------------------------------------------------------------ */
+typedef struct nse2d_les* nse2d_les_t;
+/* fine, coarse: channel descriptors of the DNS and LES grids (metric arrays borrowed), with
+ * Nx_c a multiple of 64, Nx_f = rx Nx_c, Ny_f - 1 = ry (Ny_c - 1), ry 1 or even, same Lx and h.
+ * max_pairs: capacity of the device-resident training set; max_batch: CNN batch capacity. */
+nse2d_les_t nse2d_les_create(const nse2d_channel_desc* fine, const nse2d_channel_desc* coarse,
+                             int max_pairs, int max_batch, unsigned long long seed, hipStream_t s);
+void nse2d_les_destroy(nse2d_les_t L);
+/* the closure's CNN (parameters, direct forward/backward) */
+nse2d_cnn_t nse2d_les_cnn(nse2d_les_t L);
+/* d_coarse [Ny_c][Nx_c] = F(d_fine [Ny_f][Nx_f]). Returns 0. */
+int nse2d_les_filter(nse2d_les_t L, const double* d_fine, double* d_coarse);
+/* From a fine state (omega_f, psi_f): the coarse state (omega_c with Thom wall rows, psi_c) and
+ * the closure target (fp64, 0 on the wall rows); any output may be NULL. append != 0 adds the
+ * pair (raw inputs, target) to the training set. 0 on success, -1 when the set is full. */
+int nse2d_les_pair(nse2d_les_t L, const double* d_omega_f, const double* d_psi_f,
+                   double* d_omega_c, double* d_psi_c, double* d_target, int append);
+int nse2d_les_num_pairs(nse2d_les_t L);
+void nse2d_les_clear_pairs(nse2d_les_t L);
+/* Normalisation: mean and standard deviation of each input over pairs [0, n) and of the target
+ * over their interior rows. get/set: 5 values each (omega, psi, u, v, target). */
+int nse2d_les_fit_scales(nse2d_les_t L, int n);
+void nse2d_les_get_scales(nse2d_les_t L, double* mean5, double* std5);
+int nse2d_les_set_scales(nse2d_les_t L, const double* mean5, const double* std5);
+/* Adam on mini-batches of `batch` pairs drawn without replacement from [0, n_train), reshuffled
+ * each epoch (seeded), learning rate lr0 with cosine decay to lr0/20; losses[step] (may be NULL)
+ * receives each step's normalised MSE. Returns the mean loss of the last 10 steps, -1 on error. */
+double nse2d_les_train(nse2d_les_t L, int n_train, int steps, int batch, float lr0,
+                       unsigned long long seed, float* losses);
+/* Normalised MSE over pairs [first, first+count) of the CNN and of a zero closure. 0 on success. */
+int nse2d_les_eval(nse2d_les_t L, int first, int count, double* loss_cnn, double* loss_zero);
+/* The closure term of one coarse state into d_out ([Ny_c][Nx_c] fp32, 0 on the wall rows). */
+int nse2d_les_closure(nse2d_les_t L, const double* d_omega_c, const double* d_psi_c, float* d_out);
+/* on != 0: the coarse channel adds the closure of each stage state to its RHS (section 5,
+ * nse2d_channel_set_closure); on = 0 detaches it. */
+int nse2d_les_attach(nse2d_les_t L, nse2d_channel_t coarse_ch, int on);
 
 /* ------------------------------------------------------------------------------------------
  * 6. Runtime helpers (device memory, streams) so that callers without another HIP binding

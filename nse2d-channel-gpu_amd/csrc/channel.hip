@@ -34,6 +34,7 @@ struct StageArgs {
     double dt_val;
     int mms;
     MmsParams mp;
+    const float* closure;  // optional additive RHS term of the interior rows ([Ny][Nx]), or nullptr
 };
 
 constexpr int SX = 64, SY = 4, RY = 8;
@@ -90,6 +91,7 @@ __global__ void __launch_bounds__(256) stage_kernel(StageArgs a) {
         double rhs = -J + a.nu * lap + a.f_row[j] - a.lin_drag * wc;
         if (MMS) rhs += mms_f_omega(a.mp, ix, a.y_node[j]);
         const size_t c = idx2(j, ix, Nx);
+        if (a.closure && j > 0 && j < Ny - 1) rhs += (double)a.closure[c];
         if (a.rhs_out) a.rhs_out[c] = rhs;
         double o;
         if (STAGE == 1) o = wc + dt * rhs;
@@ -282,6 +284,10 @@ struct ChannelImpl {
     std::vector<std::pair<int, size_t>> ev_marks;  // (class, index of the start event)
     double prof_ms[NSE2D_PROF_NCLASS] = {0};
     long long prof_n[NSE2D_PROF_NCLASS] = {0};
+    // optional closure term: before each stage closure_fn fills closure_buf from the stage state
+    nse2d_closure_fn closure_fn = nullptr;
+    void* closure_ctx = nullptr;
+    float* closure_buf = nullptr;
     double* t_ptr() { return scalars + 0; }
     double* dt_ptr() { return scalars + 1; }
     double* max_ptr() { return scalars + 2; }
@@ -324,6 +330,7 @@ StageArgs stage_args(ChannelImpl* c, const double* ws, const double* wn, double*
     a.a_node = d.d_a_node; a.a_edge = d.d_a_edge; a.f_row = c->f_row; a.y_node = d.d_y_node;
     a.dt_ptr = dt_ptr; a.dt_val = dt_val;
     a.mms = d.mms;
+    a.closure = c->closure_fn ? c->closure_buf : nullptr;
     if (d.mms) a.mp = make_mms_params(d.Nx, d.Lx, d.h, t_mms, d.nu, d.lin_drag, d.mms_A0, d.mms_eps, d.mms_Om, d.mms_kx);
     else a.mp = MmsParams{};
     return a;
@@ -353,7 +360,8 @@ int stage_tile_code() {
 
 template <int STAGE, bool MMS>
 void launch_stage_t(ChannelImpl* c, const StageArgs& a) {
-    const int code = (!MMS && c->d.Nx % 64 == 0) ? c->stage_tile : 0;
+    // the manufactured forcing and the closure term run in the generic kernel
+    const int code = (!MMS && !a.closure && c->d.Nx % 64 == 0) ? c->stage_tile : 0;
 #define NSE2D_STAGE_WAVE(RYV, SYV, PFV)                                                                  \
     case RYV * 100 + SYV * 10 + PFV:                                                                     \
         stage_kernel_wave<STAGE, RYV, SYV, PFV><<<(c->d.Nx / 64) * ceil_div(c->d.Ny, RYV * SYV),         \
@@ -401,13 +409,21 @@ void close_stage(ChannelImpl* c, double* f, int stage, bool bookkeeping) {
     }
 }
 
+// The closure term of the stage state (omega_s, psi), when one is attached.
+void closure_term(ChannelImpl* c, const double* ws) {
+    if (c->closure_fn) c->closure_fn(c->closure_ctx, ws, c->psi, c->closure_buf, c->s);
+}
+
 void do_step(ChannelImpl* c, double t, double dt, const double* dt_dev, int flags) {
     const bool book = (flags & NSE2D_STEP_WALL_BOOKKEEPING) != 0;
     double* rhs3 = (flags & NSE2D_STEP_STORE_RHS) ? c->rhs : nullptr;
+    closure_term(c, c->w);
     launch_stage<1>(c, stage_args(c, c->w, nullptr, c->w1, nullptr, dt_dev, dt, t));
     close_stage(c, c->w1, 0, book);
+    closure_term(c, c->w1);
     launch_stage<2>(c, stage_args(c, c->w1, c->w, c->w2, nullptr, dt_dev, dt, t + dt));
     close_stage(c, c->w2, 1, book);
+    closure_term(c, c->w2);
     launch_stage<3>(c, stage_args(c, c->w2, c->w, c->w, rhs3, dt_dev, dt, t + 0.5 * dt));
     close_stage(c, c->w, 2, book);
 }
@@ -859,6 +875,7 @@ void nse2d_channel_destroy(nse2d_channel_t ch) {
     ChannelImpl* c = &ch->impl;
     (void)hipStreamSynchronize(c->s);
     for (double* f : {c->w, c->w1, c->w2, c->psi, c->rhs, c->f_row, c->scalars}) (void)hipFree(f);
+    (void)hipFree(c->closure_buf);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     poisson_destroy(c->P);
     delete ch;
@@ -933,6 +950,18 @@ void nse2d_channel_profile_read(nse2d_channel_t ch, double* ms, long long* count
         if (ms) ms[k] = c->prof_ms[k];
         if (count) count[k] = c->prof_n[k];
     }
+}
+
+int nse2d_channel_set_closure(nse2d_channel_t ch, nse2d_closure_fn fn, void* ctx) {
+    ChannelImpl* c = &ch->impl;
+    if (fn && !c->closure_buf) {
+        const size_t N = (size_t)c->d.Nx * c->d.Ny;
+        NSE2D_HIP_CHECK(hipMalloc(&c->closure_buf, N * sizeof(float)));
+        NSE2D_HIP_CHECK(hipMemsetAsync(c->closure_buf, 0, N * sizeof(float), c->s));
+    }
+    c->closure_fn = fn;
+    c->closure_ctx = fn ? ctx : nullptr;
+    return 0;
 }
 
 double* nse2d_channel_time_ptr(nse2d_channel_t ch) { return ch->impl.t_ptr(); }

@@ -609,6 +609,12 @@ double nse2d_cnn_train_step(nse2d_cnn_t c, const float* d_in, const float* d_tar
     cnn_adam(k, lr);
     return loss;
 }
+double nse2d_cnn_eval_loss(nse2d_cnn_t c, const float* d_in, const float* d_target, int B) {
+    CnnImpl* k = c->impl;
+    if (B < 1 || B > k->maxB) { set_last_error("nse2d_cnn_eval_loss: batch outside [1, max_batch]"); return -1.0; }
+    forward(k, d_in, B);
+    return cnn_loss_and_grad(k, d_target, B);
+}
 int nse2d_cnn_adam_step(nse2d_cnn_t c, float lr) {
     cnn_adam(c->impl, lr);
     return 0;

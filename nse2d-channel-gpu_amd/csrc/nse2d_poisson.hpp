@@ -36,6 +36,9 @@ bool poisson_inverse_closure(PoissonImpl* p, double* psi, double* omega, const d
 // (the register row FFT, Nx in [512, 16384]). The inverse fuses the wall closure of each field's
 // omega (d_omegas may be null: no closure).
 size_t poisson_hat_bytes(const PoissonImpl* p);
+// The plan's own spectrum buffer ([Ny][Nx/2] complex, slot 0 of a row holds (X_0, X_{Nx/2})):
+// written by poisson_forward, read by poisson_inverse.
+void* poisson_hat(PoissonImpl* p);
 int poisson_group_supported(const PoissonImpl* p);
 void poisson_forward_group(PoissonImpl* p, const double* const* d_omegas, void* const* d_hats, int n,
                            hipStream_t s);

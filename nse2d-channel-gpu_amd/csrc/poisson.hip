@@ -678,6 +678,7 @@ void poisson_inverse(PoissonImpl* p, double* psi, hipStream_t s) {
 // ---- group launches: n equally sized fields that share plan p's coefficients, each with its own
 // spectrum buffer (poisson_hat_bytes). Pointer tables are device arrays of n pointers.
 size_t poisson_hat_bytes(const PoissonImpl* p) { return sizeof(cplx) * (size_t)p->Ny * p->M; }
+void* poisson_hat(PoissonImpl* p) { return p->d_hat; }
 
 template <int M, int R>
 void launch_rows_forward_group(const double* const* om, cplx* const* hats, int n, int rows, const cplx* twp,

@@ -93,6 +93,23 @@ SIGNATURES = {
     "nse2d_cnn_backward": (_i, [_vp, _vp, _i]),
     "nse2d_cnn_train_step": (_d, [_vp, _vp, _vp, _i, ctypes.c_float]),
     "nse2d_cnn_adam_step": (_i, [_vp, ctypes.c_float]),
+    "nse2d_cnn_eval_loss": (_d, [_vp, _vp, _vp, _i]),
+    # learned closure of a coarse channel (LES)
+    "nse2d_channel_set_closure": (_i, [_vp, _vp, _vp]),
+    "nse2d_les_create": (_vp, [_vp, _vp, _i, _i, ctypes.c_ulonglong, _stream]),
+    "nse2d_les_destroy": (None, [_vp]),
+    "nse2d_les_cnn": (_vp, [_vp]),
+    "nse2d_les_filter": (_i, [_vp, _vp, _vp]),
+    "nse2d_les_pair": (_i, [_vp, _vp, _vp, _vp, _vp, _vp, _i]),
+    "nse2d_les_num_pairs": (_i, [_vp]),
+    "nse2d_les_clear_pairs": (None, [_vp]),
+    "nse2d_les_fit_scales": (_i, [_vp, _i]),
+    "nse2d_les_get_scales": (None, [_vp, _p, _p]),
+    "nse2d_les_set_scales": (_i, [_vp, _p, _p]),
+    "nse2d_les_train": (_d, [_vp, _i, _i, _i, ctypes.c_float, ctypes.c_ulonglong, ctypes.POINTER(ctypes.c_float)]),
+    "nse2d_les_eval": (_i, [_vp, _i, _i, _p, _p]),
+    "nse2d_les_closure": (_i, [_vp, _vp, _vp, _vp]),
+    "nse2d_les_attach": (_i, [_vp, _vp, _i]),
     # ensembles
     "nse2d_ensemble_create": (_vp, [_vp, _i, _stream]),
     "nse2d_ensemble_destroy": (None, [_vp]),

@@ -13,7 +13,7 @@ HEADER = os.path.join(ROOT, "include", "nse2d_hip.h")
 def declared_functions():
     src = open(HEADER).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    names = re.findall(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\**\s+\**\s*([A-Za-z_][A-Za-z0-9_]*)\s*\(",
+    names = re.findall(r"^\s*(?!typedef\b)(?:const\s+)?[A-Za-z_][A-Za-z0-9_]*\s*\**\s+\**\s*([A-Za-z_][A-Za-z0-9_]*)\s*\(",
                        src, flags=re.M)
     return sorted(set(n for n in names if n not in ("if", "return")))
 

@@ -1,0 +1,251 @@
+"""The learned-closure LES path (csrc/les.hip, nse2d_les_* and nse2d_channel_set_closure) on the GPU:
+
+- the filter F (box in mapped y, sharp cut in x) against the numpy oracle (oracle/les_oracle.py),
+  1e-13 normwise;
+- the training pair (coarse state with Thom walls, closure target F(R_f) - R_c) against the same
+  composition of the FP64 oracle's solve, closure, Jacobian, Laplacian and RHS, and the identity
+  target = 0 when the coarse grid is the fine grid;
+- the stage hook: one SSPRK(3,3) step with a fixed additive source against the oracle's step
+  with the same source (the step bars of test_gpu_step);
+- the closure term against the CNN oracle (bf16-emulating) applied to the normalised inputs;
+- training on filtered DNS: the trained network's hold-out error against a zero closure (the gain
+  of the learned part a priori), and coarse runs with and without it against the filtered DNS
+  (the gain a posteriori).
+"""
+import ctypes
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import nse2d_hip as nh
+import oracle
+from gpu_util import Grid, rel_err, smooth_random_field, sync
+from oracle import P
+from test_gpu_step import OracleStepper, field, make_channel, step_errors
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import cnn_oracle as co  # noqa: E402
+import les_oracle as lo  # noqa: E402
+
+import les_experiment as lx  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+L2PI = 2 * math.pi
+
+
+def cfg(Nx, Ny, Reb=5000.0, **kw):
+    c = dict(Nx=Nx, Ny=Ny, Lx=L2PI, h=1.0, beta=2.0, Ub=1.0, Reb=Reb, F0=0.05, nforce=12, drag=1e-3, amp=0.05,
+             band_kmax=8, seed=11, cfl=0.5, cvisc=0.5, dt_max=1e-2)
+    c.update(kw)
+    return c
+
+
+def oracle_rhs(g, psi, w, nu, F0, nforce, drag):
+    L = oracle.lib()
+    J, lap, r = np.zeros_like(w), np.zeros_like(w), np.zeros_like(w)
+    L.orc_arakawa_J_mapped(P(psi), P(w), P(J), g.Nx, g.Ny, g.inv_dx, g.inv_deta, P(g.m.a_node))
+    L.orc_laplacian_fd_mapped(P(w), P(lap), g.Nx, g.Ny, g.inv_dx2, g.inv_deta2, P(g.m.a_node), P(g.m.a_edge))
+    L.orc_rhs_mapped(P(J), P(lap), P(w), P(r), g.Nx, g.Ny, nu, F0, nforce, g.h, drag, P(g.m.y))
+    return r
+
+
+def fine_state(g, psi_top, seed):
+    """A fine state as the solver holds it: psi = solve(omega), omega's wall rows by Thom."""
+    w = smooth_random_field(g.Nx, g.Ny, seed, kmax=6, y=g.m.y) + 3.0 * g.m.y[:, None]
+    psi = oracle.poisson_solve(w, g.Lx, g.m, 0.0, psi_top)
+    oracle.lib().orc_thom_wall_vorticity_mapped(P(w), P(psi), g.Nx, g.Ny, P(g.m.dy_edge), 0.0, psi_top, 0)
+    return w, psi
+
+
+@pytest.mark.parametrize("fine,coarse", [((256, 129), (64, 33)), ((512, 257), (64, 33)), ((128, 65), (64, 65))])
+def test_filter_matches_oracle(fine, coarse):
+    print(f"[les] filter {fine} -> {coarse}", flush=True)
+    F = lx.Channel(cfg(*fine), ic=False)
+    C = lx.Channel(cfg(*coarse), ic=False)
+    les = lx.Les(F, C, 1, 1)
+    rx, ry = fine[0] // coarse[0], (fine[1] - 1) // (coarse[1] - 1)
+    f = smooth_random_field(fine[0], fine[1], 5, kmax=12) + np.random.default_rng(1).normal(size=(fine[1], fine[0]))
+    d_f, d_c = nh.DeviceBuffer.from_array(f), nh.DeviceBuffer(coarse[0] * coarse[1])
+    assert nh.lib().nse2d_les_filter(les.h, d_f.ptr, d_c.ptr) == 0
+    got = d_c.download((coarse[1], coarse[0]))
+    ref = lo.filter_field(f, rx, ry)
+    e = rel_err(got, ref)
+    print(f"[les] filter rel err {e:.2e}", flush=True)
+    les.destroy()
+    F.destroy()
+    C.destroy()
+    assert e <= 1e-13
+
+
+@pytest.mark.parametrize("fine,coarse", [((256, 129), (64, 33)), ((512, 257), (64, 33))])
+def test_pair_matches_oracle(fine, coarse):
+    print(f"[les] pair {fine} -> {coarse}", flush=True)
+    cf, cc = cfg(*fine), cfg(*coarse)
+    F = lx.Channel(cf, ic=False)
+    C = lx.Channel(cc, ic=False)
+    les = lx.Les(F, C, 2, 1)
+    gf, gc = Grid(fine[0], fine[1], 2.0), Grid(coarse[0], coarse[1], 2.0)
+    psi_top = 2.0
+    wf, psif = fine_state(gf, psi_top, 9)
+    nh.upload_ptr(F.field(nh.FIELD_OMEGA), wf)
+    nh.upload_ptr(F.field(nh.FIELD_PSI), psif)
+    les.pair(append=True)
+    assert les.num_pairs() == 1
+    wc, psic, tgt = (b.download((coarse[1], coarse[0])) for b in (les.wc, les.psic, les.tgt))
+    # the oracle's composition
+    rx, ry = fine[0] // coarse[0], (fine[1] - 1) // (coarse[1] - 1)
+    nu = 2.0 / cf["Reb"]
+    rf = oracle_rhs(gf, psif, wf, nu, 0.05, 12, 1e-3)
+    wc_o = np.ascontiguousarray(lo.filter_field(wf, rx, ry))
+    psic_o = oracle.poisson_solve(wc_o, gc.Lx, gc.m, 0.0, psi_top)
+    oracle.lib().orc_thom_wall_vorticity_mapped(P(wc_o), P(psic_o), gc.Nx, gc.Ny, P(gc.m.dy_edge), 0.0, psi_top, 0)
+    rc = oracle_rhs(gc, psic_o, wc_o, nu, 0.05, 12, 1e-3)
+    frf = lo.filter_field(rf, rx, ry)
+    tgt_o = np.zeros_like(frf)
+    tgt_o[1:-1] = frf[1:-1] - rc[1:-1]
+    e_w, e_psi = rel_err(wc[1:-1], wc_o[1:-1]), rel_err(psic, psic_o)
+    # the target is a difference of two RHS of similar size: its bar is relative to them
+    e_t = float(np.max(np.abs(tgt - tgt_o)) / np.max(np.abs(frf[1:-1])))
+    print(f"[les] pair: omega_c {e_w:.2e}, psi_c {e_psi:.2e}, target {e_t:.2e} "
+          f"(|target| / |F(R_f)| = {np.max(np.abs(tgt_o)) / np.max(np.abs(frf[1:-1])):.2e})", flush=True)
+    les.destroy()
+    F.destroy()
+    C.destroy()
+    assert e_w <= 1e-13 and e_psi <= 1e-12 and e_t <= 1e-11
+    assert np.all(tgt[[0, -1]] == 0.0)
+
+
+def test_target_vanishes_when_the_grids_coincide():
+    """rx = ry = 1: F keeps every mode below the Nyquist one. With x-modes below Nx/4 only, the
+    state and its RHS (quadratic in the state) have no Nyquist content, F is the identity on
+    both, and the target is zero up to rounding."""
+    c = cfg(64, 33)
+    F = lx.Channel(c, ic=False)
+    C = lx.Channel(c, ic=False)
+    les = lx.Les(F, C, 1, 1)
+    g = Grid(64, 33, 2.0)
+    w = smooth_random_field(64, 33, 4, kmax=6, y=g.m.y) + 3.0 * g.m.y[:, None]
+    Wk = np.fft.rfft(w, axis=1)
+    Wk[:, 64 // 4:] = 0.0
+    wf = np.ascontiguousarray(np.fft.irfft(Wk, n=64, axis=1))
+    psif = oracle.poisson_solve(wf, g.Lx, g.m, 0.0, 2.0)
+    oracle.lib().orc_thom_wall_vorticity_mapped(P(wf), P(psif), g.Nx, g.Ny, P(g.m.dy_edge), 0.0, 2.0, 0)
+    nh.upload_ptr(F.field(nh.FIELD_OMEGA), wf)
+    nh.upload_ptr(F.field(nh.FIELD_PSI), psif)
+    les.pair(append=False)
+    tgt = les.tgt.download((33, 64))
+    rf = oracle_rhs(g, psif, wf, 2.0 / c["Reb"], 0.05, 12, 1e-3)
+    r = float(np.max(np.abs(tgt)) / np.max(np.abs(rf[1:-1])))
+    print(f"[les] identity target {r:.2e}", flush=True)
+    les.destroy()
+    F.destroy()
+    C.destroy()
+    assert r <= 1e-10
+
+
+class SourceStepper(OracleStepper):
+    """The oracle's step with a fixed additive source on the interior rows."""
+
+    def __init__(self, *a, source=None, **kw):
+        super().__init__(*a, **kw)
+        self.source = source
+
+    def build_rhs(self, psi, w, t):
+        r = super().build_rhs(psi, w, t)
+        r[1:-1] += self.source[1:-1]
+        return r
+
+
+CLOSURE_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                              ctypes.c_void_p)
+
+
+@pytest.mark.parametrize("Nx,Ny", [(64, 33), (256, 77)])
+def test_stage_hook_adds_the_source(Nx, Ny):
+    print(f"[les] stage hook {Nx}x{Ny}", flush=True)
+    g = Grid(Nx, Ny, 2.0)
+    nu, F0, nforce, drag, psi_top, dt = 2e-3, 0.05, 12, 1e-3, 2.0, 2e-4
+    src = (0.3 * smooth_random_field(Nx, Ny, 21, kmax=5, y=g.m.y)).astype(np.float32)
+    d_src = nh.DeviceArray(src)
+    calls = []
+
+    def hook(ctx, w, psi, out, stream):
+        calls.append(1)
+        nh.lib().nse2d_memcpy_d2d(out, d_src.ptr, src.nbytes)
+
+    cb = CLOSURE_FN(hook)
+    w0 = smooth_random_field(Nx, Ny, 3, kmax=6, y=g.m.y) + 3.0 * g.m.y[:, None]
+    o = SourceStepper(g, nu, F0, nforce, drag, psi_top, source=src.astype(np.float64))
+    w_o = w0.copy()
+    psi_o = o.solve(w_o)
+    ch = make_channel(g, nu, F0, nforce, drag, psi_top)
+    assert nh.lib().nse2d_channel_set_closure(ch, ctypes.cast(cb, ctypes.c_void_p), None) == 0
+    nh.upload_ptr(nh.lib().nse2d_channel_field(ch, nh.FIELD_OMEGA), w0)
+    nh.lib().nse2d_channel_init(ch)
+    psi_o, w_o = o.step(psi_o, w_o, 0.0, dt)
+    nh.lib().nse2d_channel_step(ch, 0.0, dt, 0)
+    sync()
+    e = step_errors(g, field(ch, nh.FIELD_OMEGA, g), field(ch, nh.FIELD_PSI, g), w_o, psi_o, psi_top)
+    nh.lib().nse2d_channel_set_closure(ch, None, None)
+    nh.lib().nse2d_channel_destroy(ch)
+    print(f"[les] hook step: psi {e[0]:.2e}, interior {e[1]:.2e}, wall vs own closure {e[2]:.2e}", flush=True)
+    assert len(calls) == 3
+    assert e[0] <= 2e-13 and e[1] <= 2e-13 and e[2] <= 1e-13
+
+
+def test_closure_term_matches_the_cnn_oracle():
+    c = cfg(64, 33)
+    F = lx.Channel(cfg(128, 65), ic=False)
+    C = lx.Channel(c, ic=False)
+    les = lx.Les(F, C, 1, 1, seed=3)
+    g = Grid(64, 33, 2.0)
+    w, psi = fine_state(g, 2.0, 6)
+    mean = np.array([0.5, 1.0, 0.7, 0.0, 0.1])
+    std = np.array([3.0, 0.6, 0.5, 0.2, 2.0])
+    assert nh.lib().nse2d_les_set_scales(les.h, mean.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                         std.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) == 0
+    d_w, d_psi = nh.DeviceBuffer.from_array(w), nh.DeviceBuffer.from_array(psi)
+    d_out = nh.DeviceArray(shape=(33, 64), dtype=np.float32)
+    assert nh.lib().nse2d_les_closure(les.h, d_w.ptr, d_psi.ptr, d_out.ptr) == 0
+    got = d_out.download().astype(np.float64)
+    cnn = nh.lib().nse2d_les_cnn(les.h)
+    p = np.zeros(nh.lib().nse2d_cnn_num_params(cnn), np.float32)
+    nh.lib().nse2d_cnn_get_params(cnn, p.ctypes.data)
+    # inputs as in csrc/les.hip: omega, psi, u, v (ops.cu:47-63), normalised, rounded to fp32
+    u, v = np.zeros_like(w), np.zeros_like(w)
+    oracle.lib().orc_uv_from_psi_mapped(P(psi), P(u), P(v), g.Nx, g.Ny, g.inv_dx, g.inv_deta, P(g.m.a_node))
+    x = np.stack([(f - mean[k]) / std[k] for k, f in enumerate((w, psi, u, v))], axis=-1)
+    x = x.astype(np.float32).astype(np.float64)[None]
+    ref, _ = co.forward(p.astype(np.float64), x, emulate=True)
+    ref = ref[0] * std[4] + mean[4]
+    ref[[0, -1]] = 0.0
+    e = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+    print(f"[les] closure term vs CNN oracle {e:.2e}", flush=True)
+    les.destroy()
+    F.destroy()
+    C.destroy()
+    assert np.all(got[[0, -1]] == 0.0)
+    assert e <= 5e-3
+
+
+def test_training_on_filtered_dns_beats_the_zero_closure():
+    """A small DNS (512x257, Re_b = 5000), 64 pairs filtered 8x8 to 64x33, 300 Adam steps: the
+    trained closure must explain most of the hold-out target (normalised MSE below half that of
+    a zero closure), coarse runs with and without it must stay finite over the window, and the
+    closure must bring the coarse run closer to the filtered DNS."""
+    print("[les] small experiment 512x257 -> 64x33", flush=True)
+    r = lx.run(cfg(512, 257), cfg(64, 33), spinup_time=1.0, window_time=1.0, n_pairs=64, holdout=8,
+               train_steps=300, batch=8, lr=2e-3, n_samples=16, log=lambda s: print("[les] " + s, flush=True))
+    print(f"[les] result {r}", flush=True)
+    assert r["holdout_mse_cnn"] < 0.5 * r["holdout_mse_zero"]
+    assert r["runs"]["closure"]["finite"] and r["runs"]["no_closure"]["finite"]
+    assert r["train_loss_first_last"][1] < r["train_loss_first_last"][0]
+    # a posteriori: started from the same filtered state, the coarse run with the closure follows
+    # the filtered DNS more closely than the run without it (measured: 0.2 % against 3.9 % at
+    # the end of the window), and its Re_tau is closer to the filtered DNS's
+    cl, nc = r["runs"]["closure"], r["runs"]["no_closure"]
+    assert cl["traj_err_end"] < 0.5 * nc["traj_err_end"]
+    assert cl["re_tau_rel_err"] < nc["re_tau_rel_err"]
