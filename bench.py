@@ -206,10 +206,43 @@ def run_ensemble(a):
     print(json.dumps(out), flush=True)
 
 
+def run_les(a):
+    """One line of JSON: the learned-closure LES at 512x257 against the 4096x2049 DNS it is trained
+    on (DNS -> 2000 filtered pairs -> Adam, batch 32 -> coarse runs with and without the closure
+    from the filtered DNS state): wall-clock speedup per unit of simulated time, Re_tau and U(y)
+    errors against the filtered DNS over the same window."""
+    import les_experiment as lx
+    fine = dict(CFG)
+    coarse = dict(CFG, Nx=512, Ny=257)
+    r = lx.run(fine, coarse, spinup_time=a.les_spinup, window_time=a.les_window, n_pairs=a.les_pairs,
+               holdout=max(1, a.les_pairs // 20), train_steps=a.les_train_steps, batch=32, lr=1e-3,
+               n_samples=min(100, a.les_pairs), log=lambda m: print("les: " + m, file=sys.stderr, flush=True))
+    cl, nc = r["runs"]["closure"], r["runs"]["no_closure"]
+    out = {
+        "metric": "LES wall-clock speedup over DNS per unit simulated time (512x257 closure LES vs 4096x2049 DNS)",
+        "value": r["speedup_vs_dns"], "unit": "x", "n_gpus": 1, "higher_is_better": True, "vs_baseline": None,
+        "dtype": "f64 solver, bf16/fp32 CNN", "data": "synthetic (DNS generated in the run)",
+        "config": {"workload": "learned-closure LES 512x257 (tanh beta=2, Re_b=2e4, F0=0.05 n=12, drag 1e-3) "
+                               "trained on 4096x2049 DNS filtered 8x in x (sharp spectral) and 8x in mapped y (box)",
+                   "pairs": a.les_pairs, "batch": 32, "train_steps": a.les_train_steps,
+                   "spinup_time": a.les_spinup, "window_time": a.les_window},
+        "re_tau_err_closure": cl.get("re_tau_rel_err"), "re_tau_err_no_closure": nc.get("re_tau_rel_err"),
+        "U_err_closure": cl.get("U_rel_err"), "U_err_no_closure": nc.get("U_rel_err"),
+        "traj_err_closure": cl.get("traj_err_end"), "traj_err_no_closure": nc.get("traj_err_end"),
+        "details": r,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["channel", "ensemble"], default="channel",
-                    help="channel: the 4096x2049 forced channel (default); ensemble: 64 independent channels")
+    ap.add_argument("--workload", choices=["channel", "ensemble", "les"], default="channel",
+                    help="channel: the 4096x2049 forced channel (default); ensemble: 64 independent channels; "
+                         "les: learned-closure LES at 512x257 trained on the 4096x2049 DNS")
+    ap.add_argument("--les-spinup", type=float, default=4.0)
+    ap.add_argument("--les-window", type=float, default=4.0)
+    ap.add_argument("--les-pairs", type=int, default=2000)
+    ap.add_argument("--les-train-steps", type=int, default=1250)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
     ap.add_argument("--warmup", type=int, default=200)
@@ -235,6 +268,9 @@ def main():
         dst.free()
     if a.workload == "ensemble":
         run_ensemble(a)
+        return
+    if a.workload == "les":
+        run_les(a)
         return
     c = CFG
     Nx, Ny = c["Nx"], c["Ny"]

@@ -249,6 +249,8 @@ def run(fine_cfg, coarse_cfg, *, spinup_time, window_time, n_pairs, holdout, tra
                 L.nse2d_device_sync()
                 dns_wall += time.perf_counter() - ta
             les.pair(append=True)
+            if k > 0 and k % 500 == 0:
+                log(f"{k} pairs, t={fine.time():.3f}")
             if k % every == 0:
                 wc = les.wc.download((coarse_cl.Ny, coarse_cl.Nx))
                 re_c = math.sqrt(0.5 * nu_c * (abs(wc[0].mean()) + abs(wc[-1].mean()))) * coarse_cl.c["h"] / nu_c
