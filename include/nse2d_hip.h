@@ -330,6 +330,12 @@ void nse2d_cnn_set_params(nse2d_cnn_t c, const float* host);
 void nse2d_cnn_get_grads(nse2d_cnn_t c, float* host);
 /* d_in: [B][H][W][4] fp32 (normalised inputs); d_out: [B][H][W] fp32 (may be NULL). 0 on success. */
 int nse2d_cnn_forward(nse2d_cnn_t c, const float* d_in, float* d_out, int B);
+/* The forward of nse2d_cnn_forward without keeping the pre-activations (inference: no backward
+ * may follow). 0 on success. */
+int nse2d_cnn_infer(nse2d_cnn_t c, const float* d_in, float* d_out, int B);
+/* on != 0: convolutions by the kernel that reads its operands straight from global memory
+ * instead of the LDS-staged one (for cross-checks; also NSE2D_CNN_DIRECT=1 at creation). */
+int nse2d_cnn_set_direct(nse2d_cnn_t c, int on);
 /* Gradients of sum(d_gout * out) for the batch of the preceding forward. 0 on success. */
 int nse2d_cnn_backward(nse2d_cnn_t c, const float* d_gout, int B);
 /* forward, mean squared error over rows 1..H-2 against d_target [B][H][W], backward, Adam

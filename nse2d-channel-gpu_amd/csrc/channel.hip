@@ -188,6 +188,109 @@ __device__ __forceinline__ void stage_wave_body(const StageArgs& a, int tile) {
     }
 }
 
+// The same stage for Nx a multiple of 128 with two columns per lane: a wavefront owns a 128-column
+// strip, lane l holds columns x0 + 2l and x0 + 2l + 1, and every row moves as one 16-byte access
+// per lane (1 KB per wavefront instruction). The x-neighbours outside the pair come from the DPP
+// shifts of the other element (lane l-1's right value, lane l+1's left value), the strip's halo
+// columns from lanes 0 and 63.
+template <int STAGE, int RY, int SYW, int PF>
+__device__ __forceinline__ void stage_wave2_body(const StageArgs& a, int tile) {
+    const int Nx = a.Nx, Ny = a.Ny;
+    const int lane = threadIdx.x;
+    const int ntx = Nx >> 7;
+    const int x0 = (tile % ntx) << 7, ix = x0 + 2 * lane;
+    const int jlo = ((tile / ntx) * SYW + threadIdx.y) * RY;
+    if (jlo >= Ny) return;  // uniform over the wavefront
+    const int jhi = min(jlo + RY, Ny);
+    const bool edge = (lane == 0) || (lane == 63);
+    const int hx = (lane == 0) ? (x0 == 0 ? Nx - 1 : x0 - 1) : (x0 + 128 == Nx ? 0 : x0 + 128);
+    const double inv4dxde = 0.25 * a.inv_dx * a.inv_deta;
+    const double dt = a.dt_ptr ? *a.dt_ptr : a.dt_val;
+    struct Raw { double2 p, w; double ph, wh; };
+    auto load = [&](int j, Raw& r) {
+        const double* __restrict__ pr = a.psi + (size_t)j * Nx;
+        const double* __restrict__ wr = a.ws + (size_t)j * Nx;
+        r.p = *reinterpret_cast<const double2*>(pr + ix);
+        r.w = *reinterpret_cast<const double2*>(wr + ix);
+        r.ph = 0.0;
+        r.wh = 0.0;
+        if (edge) { r.ph = pr[hx]; r.wh = wr[hx]; }
+    };
+    // (x0+2l-1, x0+2l, x0+2l+1, x0+2l+2) of a loaded row
+    struct Row4 { double v[4]; };
+    auto expand = [&](const Raw& r, Row4& P4, Row4& W4) {
+        P4.v[0] = wave_shr1(r.p.y, r.ph); P4.v[1] = r.p.x; P4.v[2] = r.p.y; P4.v[3] = wave_shl1(r.p.x, r.ph);
+        W4.v[0] = wave_shr1(r.w.y, r.wh); W4.v[1] = r.w.x; W4.v[2] = r.w.y; W4.v[3] = wave_shl1(r.w.x, r.wh);
+    };
+    auto stencil = [&](const Row4& m, const Row4& c, const Row4& pl, int e) {
+        Stencil9 S;
+        S.xm = c.v[e]; S.c = c.v[e + 1]; S.xp = c.v[e + 2];
+        S.mm = m.v[e]; S.ym = m.v[e + 1]; S.pm = m.v[e + 2];
+        S.mp = pl.v[e]; S.yp = pl.v[e + 1]; S.pp = pl.v[e + 2];
+        return S;
+    };
+    Row4 Pm{}, Pc{}, Pp{}, Wm{}, Wc{}, Wp{};
+    Raw ring[PF];
+    double2 wring[PF];
+#pragma unroll
+    for (int f = 0; f < PF; ++f) {
+        if (f < RY && jlo + 1 + f < Ny) load(jlo + 1 + f, ring[f]);
+        if (STAGE > 1 && f < RY && jlo + f < jhi)
+            wring[f] = *reinterpret_cast<const double2*>(a.wn + idx2(jlo + f, ix, Nx));
+    }
+    if (jlo > 0) {
+        Raw r;
+        load(jlo - 1, r);
+        expand(r, Pm, Wm);
+    }
+    {
+        Raw r;
+        load(jlo, r);
+        expand(r, Pc, Wc);
+    }
+#pragma unroll
+    for (int q = 0; q < RY; ++q) {
+        const int j = jlo + q;
+        if (j >= jhi) break;
+        const Raw cur = ring[q % PF];  // row j+1
+        const double2 wn_c = (STAGE > 1) ? wring[q % PF] : double2{0.0, 0.0};
+        if (q + PF < RY && j + 1 + PF < Ny) load(j + 1 + PF, ring[q % PF]);
+        if (STAGE > 1 && q + PF < RY && j + PF < jhi)
+            wring[q % PF] = *reinterpret_cast<const double2*>(a.wn + idx2(j + PF, ix, Nx));
+        if (j + 1 < Ny) expand(cur, Pp, Wp);
+        double o[2];
+        const bool interior = (j > 0 && j < Ny - 1);
+        const double an = interior ? a.a_node[j] : 0.0;
+        const double ae = interior ? a.a_edge[j] : 0.0, aw = interior ? a.a_edge[j - 1] : 0.0;
+        const double fr = a.f_row[j];
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+            double J = 0.0, lap = 0.0;
+            const Stencil9 P = stencil(Pm, Pc, Pp, e), W = stencil(Wm, Wc, Wp, e);
+            if (interior) {
+                double J1, J2, J3;
+                arakawa_terms(P, W, an, inv4dxde, J1, J2, J3);
+                J = -((J1 + J2 + J3) * (1.0 / 3.0));
+                lap = mapped_laplacian(W.xm, W.c, W.xp, W.ym, W.yp, an, ae, aw, a.inv_dx2, a.inv_deta2);
+            }
+            const double wc = W.c;
+            const double rhs = -J + a.nu * lap + fr - a.lin_drag * wc;
+            if (a.rhs_out) a.rhs_out[idx2(j, ix + e, Nx)] = rhs;
+            const double wn = e ? wn_c.y : wn_c.x;
+            if (STAGE == 1) o[e] = wc + dt * rhs;
+            else if (STAGE == 2) o[e] = 0.75 * wn + 0.25 * (wc + dt * rhs);
+            else o[e] = (1.0 / 3.0) * wn + (2.0 / 3.0) * (wc + dt * rhs);
+        }
+        *reinterpret_cast<double2*>(a.out + idx2(j, ix, Nx)) = double2{o[0], o[1]};
+        Pm = Pc; Pc = Pp; Wm = Wc; Wc = Wp;
+    }
+}
+
+template <int STAGE, int RY, int SYW, int PF>
+__global__ void __launch_bounds__(64 * SYW) stage_kernel_wave2(StageArgs a) {
+    stage_wave2_body<STAGE, RY, SYW, PF>(a, xcd_tile(blockIdx.x, gridDim.x));
+}
+
 template <int STAGE, int RY, int SYW, int PF>
 __global__ void __launch_bounds__(64 * SYW) stage_kernel_wave(StageArgs a) {
     stage_wave_body<STAGE, RY, SYW, PF>(a, xcd_tile(blockIdx.x, gridDim.x));  // tiles contiguous per XCD
@@ -252,6 +355,41 @@ __global__ void dt_from_maxima_kernel(const double* __restrict__ m, double* __re
     if (threadIdx.x == 0) *dt_out = dt_rule(m, r);
 }
 
+// The CFL maxima of psi (diag.cu:702-866), the dt rule (main.cu:451-458) and the time advance in
+// two launches: Ny workgroups reduce one row each to its three maxima (cfl_rows_kernel), then one
+// workgroup reduces the Ny partials in row order and writes the maxima (scal[2..4]), dt (scal[1])
+// and t + dt (scal[0]) (cfl_final_kernel). A single launch that lets its last workgroup finish
+// needs a device-scope release fence in every workgroup, which writes back the XCD's L2 each time
+// (measured 2x slower than the two launches).
+constexpr int kCflThreads = 256;
+__global__ void __launch_bounds__(kCflThreads) cfl_rows_kernel(const double* __restrict__ psi, int Nx, int Ny,
+                                                               double inv_dx, double inv_deta,
+                                                               const double* __restrict__ a_node,
+                                                               double* __restrict__ partial) {
+    __shared__ double sm[3 * 16];
+    row_max_uv_body(psi, Nx, Ny, inv_dx, inv_deta, a_node, xcd_tile(blockIdx.x, gridDim.x), partial, sm);
+}
+__global__ void __launch_bounds__(kCflThreads) cfl_final_kernel(const double* __restrict__ partial, int Ny,
+                                                                double* __restrict__ scal, DtRule r) {
+    __shared__ double sm[3 * 16];
+    double m[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        double v = 0.0;
+        for (int i = threadIdx.x; i < Ny; i += kCflThreads) v = fmax(v, partial[(size_t)a * Ny + i]);
+        m[a] = v;
+    }
+    block_max<3>(m, sm);
+    if (threadIdx.x == 0) {
+        scal[2] = m[0];
+        scal[3] = m[1];
+        scal[4] = m[2];
+        const double dt = dt_rule(m, r);
+        scal[1] = dt;
+        scal[0] = scal[0] + dt;
+    }
+}
+
 // Ensembles: one thread per member; member m's scalars live at scal[8m]: t, dt, umax, vmax, vmaxm.
 constexpr int kEnsScal = 8;
 __global__ void ens_dt_kernel(const DtRule* __restrict__ rules, double* __restrict__ scal, int n) {
@@ -277,7 +415,7 @@ struct ChannelImpl {
     PoissonImpl* P = nullptr;
     double inv_dx = 0, inv_deta = 0, inv_dx2 = 0, inv_deta2 = 0, dx = 0, deta = 0;
     // optional timing of kernel classes with events on the launch stream
-    int stage_tile = 841;  // wavefront stage kernel shape RY*100 + SYW*10 + PF (0: generic kernel)
+    int stage_tile = 10821;  // stage kernel shape (stage_tile_code; 0: generic kernel)
     int prof_mask = 0;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
@@ -288,6 +426,8 @@ struct ChannelImpl {
     nse2d_closure_fn closure_fn = nullptr;
     void* closure_ctx = nullptr;
     float* closure_buf = nullptr;
+    // per-channel row partials of the CFL maxima
+    double* cfl_partial = nullptr;
     double* t_ptr() { return scalars + 0; }
     double* dt_ptr() { return scalars + 1; }
     double* max_ptr() { return scalars + 2; }
@@ -337,31 +477,44 @@ StageArgs stage_args(ChannelImpl* c, const double* ws, const double* wn, double*
 }
 
 // Shape of the wavefront stage kernel: RY rows per wavefront, SYW wavefronts per workgroup and
-// prefetch depth PF, coded RY*100 + SYW*10 + PF. NSE2D_STAGE_TILE=RYxSYWxPF (or RYxSYW, PF = 2)
+// prefetch depth PF, coded RY*100 + SYW*10 + PF, plus 10000 for the two-column kernels.
+// NSE2D_STAGE_TILE=RYxSYWxPF (or RYxSYW, PF = 2; a leading 'w' for two columns per lane)
 // selects another instantiated shape (used to tune on the device); 0 selects the generic kernel.
+// Default: w8x2x1 (measured at 4096x2049: 150 us per step for the three stages, against 207 for
+// the best one-column shape 8x4x1: the 16-byte accesses reach the copy bandwidth).
 // The manufactured-solution forcing (sin/cos per point) always runs in the generic kernel, so the
 // production kernels carry none of its registers.
-constexpr int kStageDefault = 841;
+constexpr int kStageDefault = 841;        // one column per lane (ensembles, Nx % 128 != 0)
+constexpr int kStageDefaultWide = 10821;  // two columns per lane, 8 rows x 2 wavefronts
 int stage_tile_code() {
-    int ry = 8, sy = 4, pf = 1;
+    int ry = 8, sy = 2, pf = 1, wide = 1;
     if (const char* e = std::getenv("NSE2D_STAGE_TILE")) {
         if (e[0] == '0') return 0;
+        if (e[0] == 'w') { wide = 1; ++e; }
         const int n = std::sscanf(e, "%dx%dx%d", &ry, &sy, &pf);
         if (n == 2) pf = 2;
-        else if (n != 3) return kStageDefault;
+        else if (n != 3) return kStageDefaultWide;
     }
-    const int c = ry * 100 + sy * 10 + pf;
+    const int c = wide * 10000 + ry * 100 + sy * 10 + pf;
     switch (c) {
         case 841: case 842: case 843: case 1641: case 1642: case 1643: case 1621: case 1622: case 1623:
-        case 3211: case 3212: case 3213: return c;
-        default: return kStageDefault;
+        case 3211: case 3212: case 3213:
+        case 10841: case 10842: case 10441: case 11621: case 10821: return c;
+        default: return kStageDefaultWide;
     }
 }
 
 template <int STAGE, bool MMS>
 void launch_stage_t(ChannelImpl* c, const StageArgs& a) {
-    // the manufactured forcing and the closure term run in the generic kernel
-    const int code = (!MMS && !a.closure && c->d.Nx % 64 == 0) ? c->stage_tile : 0;
+    // the manufactured forcing and the closure term run in the generic kernel; the two-column
+    // kernels (codes >= 10000) need Nx a multiple of 128, else the one-column kernel 8x4x1 runs
+    int code = (!MMS && !a.closure && c->d.Nx % 64 == 0) ? c->stage_tile : 0;
+    if (code >= 10000 && c->d.Nx % 128 != 0) code = kStageDefault;
+#define NSE2D_STAGE_WAVE2(RYV, SYV, PFV)                                                                 \
+    case 10000 + RYV * 100 + SYV * 10 + PFV:                                                             \
+        stage_kernel_wave2<STAGE, RYV, SYV, PFV><<<(c->d.Nx / 128) * ceil_div(c->d.Ny, RYV * SYV),       \
+                                                   dim3(64, SYV), 0, c->s>>>(a);                         \
+        break;
 #define NSE2D_STAGE_WAVE(RYV, SYV, PFV)                                                                  \
     case RYV * 100 + SYV * 10 + PFV:                                                                     \
         stage_kernel_wave<STAGE, RYV, SYV, PFV><<<(c->d.Nx / 64) * ceil_div(c->d.Ny, RYV * SYV),         \
@@ -372,12 +525,15 @@ void launch_stage_t(ChannelImpl* c, const StageArgs& a) {
         NSE2D_STAGE_WAVE(16, 4, 1) NSE2D_STAGE_WAVE(16, 4, 2) NSE2D_STAGE_WAVE(16, 4, 3)
         NSE2D_STAGE_WAVE(16, 2, 1) NSE2D_STAGE_WAVE(16, 2, 2) NSE2D_STAGE_WAVE(16, 2, 3)
         NSE2D_STAGE_WAVE(32, 1, 1) NSE2D_STAGE_WAVE(32, 1, 2) NSE2D_STAGE_WAVE(32, 1, 3)
+        NSE2D_STAGE_WAVE2(8, 4, 1) NSE2D_STAGE_WAVE2(8, 4, 2) NSE2D_STAGE_WAVE2(4, 4, 1)
+        NSE2D_STAGE_WAVE2(16, 2, 1) NSE2D_STAGE_WAVE2(8, 2, 1)
         default: {
             const unsigned ntiles = ceil_div(c->d.Nx, SX) * ceil_div(c->d.Ny, SY * RY);
             stage_kernel<STAGE, MMS><<<ntiles, dim3(SX, SY), 0, c->s>>>(a);
         }
     }
 #undef NSE2D_STAGE_WAVE
+#undef NSE2D_STAGE_WAVE2
 }
 
 template <int STAGE>
@@ -390,7 +546,7 @@ void launch_stage(ChannelImpl* c, const StageArgs& a) {
 
 void close_stage(ChannelImpl* c, double* f, int stage, bool bookkeeping) {
     const nse2d_channel_desc& d = c->d;
-    { ProfScope ps(c, NSE2D_PROF_FFT_FWD); poisson_forward(c->P, f, c->s); }
+    { ProfScope ps(c, NSE2D_PROF_FFT_FWD); poisson_forward_interior(c->P, f, c->s); }
     { ProfScope ps(c, NSE2D_PROF_THOMAS); poisson_tridiag(c->P, d.psi_bot, d.psi_top, c->s); }
     if (!bookkeeping && poisson_fuses_closure(c->P)) {
         // the closure rides in the inverse transform's workgroups of rows 1 and Ny-2
@@ -710,7 +866,8 @@ EnsembleImpl* ens_create(const nse2d_channel_desc* descs, int n, hipStream_t s) 
     NSE2D_HIP_CHECK(hipMemcpyAsync(e->d_args, args.data(), sizeof(StageArgs) * 3 * n, hipMemcpyHostToDevice, e->s));
     NSE2D_HIP_CHECK(hipMemcpyAsync(e->d_jobs, jobs.data(), sizeof(MaxUVJob) * n, hipMemcpyHostToDevice, e->s));
     e->stage_tile = stage_tile_code();
-    if (e->stage_tile == 0) e->stage_tile = kStageDefault;  // the generic kernel has no group form
+    // the generic and the two-column kernels have no group form
+    if (e->stage_tile == 0 || e->stage_tile >= 10000) e->stage_tile = kStageDefault;
     ens_set_tiles(e);
     return e;
 }
@@ -862,6 +1019,7 @@ nse2d_channel_t nse2d_channel_create(const nse2d_channel_desc* desc, hipStream_t
         NSE2D_HIP_CHECK(hipMemsetAsync(*f, 0, N * sizeof(double), s));
     }
     NSE2D_HIP_CHECK(hipMalloc(&c->f_row, d.Ny * sizeof(double)));
+    NSE2D_HIP_CHECK(hipMalloc(&c->cfl_partial, 3 * (size_t)d.Ny * sizeof(double)));
     NSE2D_HIP_CHECK(hipMalloc(&c->scalars, 16 * sizeof(double)));
     NSE2D_HIP_CHECK(hipMemsetAsync(c->scalars, 0, 16 * sizeof(double), s));
     forcing_rows_kernel<<<ceil_div(d.Ny, 256), 256, 0, s>>>(c->f_row, d.d_y_node, d.Ny, d.F0, d.nforce, d.h);
@@ -876,6 +1034,7 @@ void nse2d_channel_destroy(nse2d_channel_t ch) {
     (void)hipStreamSynchronize(c->s);
     for (double* f : {c->w, c->w1, c->w2, c->psi, c->rhs, c->f_row, c->scalars}) (void)hipFree(f);
     (void)hipFree(c->closure_buf);
+    (void)hipFree(c->cfl_partial);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     poisson_destroy(c->P);
     delete ch;
@@ -912,18 +1071,26 @@ void nse2d_channel_step_auto(nse2d_channel_t ch, int n) {
         std::abort();
     }
     const double inv_dymin2 = 1.0 / (d.dy_min * d.dy_min);
+    const DtRule rule{c->dx, c->deta, d.cfl, d.cvisc, d.nu, c->inv_dx2, inv_dymin2, d.dt_max};
     for (int k = 0; k < n; ++k) {
         double* m = c->max_ptr();
         {
             ProfScope ps(c, NSE2D_PROF_CFL);
-            launch_max_uv(c->psi, d.Nx, d.Ny, c->inv_dx, c->inv_deta, d.d_a_node, m, m + 1, m + 2, c->s);
-            dt_from_maxima_kernel<<<1, 64, 0, c->s>>>(
-                m, c->dt_ptr(), DtRule{c->dx, c->deta, d.cfl, d.cvisc, d.nu, c->inv_dx2, inv_dymin2, d.dt_max});
+            if (d.Nx < 2 || d.Ny < 3) {  // diag.cu:826-832: zero maxima
+                launch_max_uv(c->psi, d.Nx, d.Ny, c->inv_dx, c->inv_deta, d.d_a_node, m, m + 1, m + 2, c->s);
+                dt_from_maxima_kernel<<<1, 64, 0, c->s>>>(m, c->dt_ptr(), rule);
+                NSE2D_LAUNCH_CHECK();
+                advance_time_kernel<<<1, 64, 0, c->s>>>(c->t_ptr(), c->dt_ptr());
+            } else {
+                // t advances before the stages: nothing in the step reads it (no manufactured forcing)
+                cfl_rows_kernel<<<d.Ny, kCflThreads, 0, c->s>>>(c->psi, d.Nx, d.Ny, c->inv_dx, c->inv_deta, d.d_a_node,
+                                                                c->cfl_partial);
+                NSE2D_LAUNCH_CHECK();
+                cfl_final_kernel<<<1, kCflThreads, 0, c->s>>>(c->cfl_partial, d.Ny, c->scalars, rule);
+            }
             NSE2D_LAUNCH_CHECK();
         }
         do_step(c, 0.0, 0.0, c->dt_ptr(), 0);
-        advance_time_kernel<<<1, 64, 0, c->s>>>(c->t_ptr(), c->dt_ptr());
-        NSE2D_LAUNCH_CHECK();
     }
 }
 

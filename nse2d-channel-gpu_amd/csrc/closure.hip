@@ -21,6 +21,7 @@
 #include "../../include/nse2d_hip.h"
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -96,10 +97,124 @@ __global__ void __launch_bounds__(256) conv_mfma_kernel(const bf16* __restrict__
         } else {
             const float z = acc[r] + bias[oc];
             const bf16 zb = (bf16)z;
-            zout[o] = zb;
+            if (zout) zout[o] = zb;  // null for inference
             if (MODE == MODE_FWD_GELU) aout[o] = (bf16)gelu((float)zb);
         }
     }
+}
+
+// The same convolution with its operands staged in LDS (the kernel used by the network): a
+// workgroup of 4 wavefronts owns 4 image rows x 64 pixels x all 64 output channels; wavefront w
+// computes row y0 + w as a 2 x 2 block of 32x32 MFMA tiles (pixels 32 pb.., channels 32 cb..),
+// so each pair of A and B fragments feeds four MFMAs. LDS holds the input halo tile (rows y0-1 ..
+// y0+4, pixels x0-1 .. x0+64, all CIN channels; rows beyond the walls are zero) and the weights of
+// one tap, double-buffered: the next tap's weights are fetched from global memory while the
+// current tap computes. Rows of both LDS arrays are padded by 8 channels (16 bytes) so that the
+// 16-byte fragment reads of consecutive lanes fall on different banks. zout may be null (no
+// pre-activations kept: inference).
+constexpr int kTileRows = 4;
+template <int CIN>
+struct ConvLds {
+    static constexpr int CP = CIN + 8;  // padded channel stride (bf16 elements)
+    bf16 in[kTileRows + 2][66][CP];
+    bf16 w[2][kC][CP];
+};
+
+template <int CIN, int MODE>
+__global__ void __launch_bounds__(256) conv_lds_kernel(const bf16* __restrict__ in, const bf16* __restrict__ w,
+                                                       const float* __restrict__ bias, bf16* __restrict__ zout,
+                                                       bf16* __restrict__ aout, const bf16* __restrict__ zprev,
+                                                       int H, int W) {
+    using S = ConvLds<CIN>;
+    constexpr int CP = S::CP, C8 = CIN / 8;
+    __shared__ alignas(16) S sh;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nseg = W >> 6, nyt = (H + kTileRows - 1) / kTileRows;
+    const int seg = blockIdx.x % nseg, yt = (blockIdx.x / nseg) % nyt, b = blockIdx.x / (nseg * nyt);
+    const int x0 = seg << 6, y0 = yt * kTileRows;
+    const int i = lane & 31, kh = lane >> 5;
+    // weights of tap t into registers (16 bytes per chunk, 64 x CIN / 8 chunks)
+    constexpr int WCH = kC * C8 / 256;  // chunks per thread (1 or 2)
+    bf16x8 wreg[WCH];
+    auto fetch_w = [&](int t) {
+#pragma unroll
+        for (int q = 0; q < WCH; ++q) {
+            const int e = tid + q * 256, n = e / C8, c8 = (e % C8) * 8;
+            wreg[q] = load8(w + ((size_t)t * kC + n) * CIN + c8);
+        }
+    };
+    auto store_w = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < WCH; ++q) {
+            const int e = tid + q * 256, n = e / C8, c8 = (e % C8) * 8;
+            *reinterpret_cast<bf16x8*>(&sh.w[buf][n][c8]) = wreg[q];
+        }
+    };
+    fetch_w(0);
+    // input halo tile
+    for (int e = tid; e < (kTileRows + 2) * 66 * C8; e += 256) {
+        const int r = e / (66 * C8), rem = e % (66 * C8), q = rem / C8, c8 = (rem % C8) * 8;
+        const int yy = y0 - 1 + r;
+        int xx = x0 - 1 + q;
+        xx = xx < 0 ? xx + W : (xx >= W ? xx - W : xx);
+        bf16x8 v;
+        if (yy >= 0 && yy < H) v = load8(in + (((size_t)b * H + yy) * W + xx) * CIN + c8);
+        else
+#pragma unroll
+            for (int k = 0; k < 8; ++k) v[k] = (bf16)0.0f;
+        *reinterpret_cast<bf16x8*>(&sh.in[r][q][c8]) = v;
+    }
+    store_w(0);
+    __syncthreads();
+    floatx16 acc[2][2];
+#pragma unroll
+    for (int pb = 0; pb < 2; ++pb)
+#pragma unroll
+        for (int cb = 0; cb < 2; ++cb) acc[pb][cb] = floatx16{0};
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+        if (t + 1 < 9) fetch_w(t + 1);
+        int dy = t / 3 - 1, dx = t % 3 - 1;
+        if (MODE == MODE_BWD_DATA) { dy = -dy; dx = -dx; }
+        const int lr = wave + 1 + dy;
+        const int buf = t & 1;
+#pragma unroll
+        for (int kb = 0; kb < CIN / 16; ++kb) {
+            const int co = kb * 16 + kh * 8;
+            const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(&sh.in[lr][i + 1 + dx][co]);
+            const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(&sh.in[lr][i + 33 + dx][co]);
+            const bf16x8 b0 = *reinterpret_cast<const bf16x8*>(&sh.w[buf][i][co]);
+            const bf16x8 b1 = *reinterpret_cast<const bf16x8*>(&sh.w[buf][i + 32][co]);
+            acc[0][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b0, acc[0][0], 0, 0, 0);
+            acc[0][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, b1, acc[0][1], 0, 0, 0);
+            acc[1][0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b0, acc[1][0], 0, 0, 0);
+            acc[1][1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, b1, acc[1][1], 0, 0, 0);
+        }
+        if (t + 1 < 9) {
+            store_w(buf ^ 1);  // the other buffer: last read in tap t-1, before the barrier below
+            __syncthreads();
+        }
+    }
+    const int y = y0 + wave;
+    if (y >= H) return;
+#pragma unroll
+    for (int pb = 0; pb < 2; ++pb)
+#pragma unroll
+        for (int cb = 0; cb < 2; ++cb)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int px = x0 + (pb << 5) + ((r >> 2) << 3) + (kh << 2) + (r & 3);
+                const int oc = (cb << 5) + i;
+                const size_t o = (((size_t)b * H + y) * W + px) * kC + oc;
+                if (MODE == MODE_BWD_DATA) {
+                    aout[o] = (bf16)(acc[pb][cb][r] * gelu_grad((float)zprev[o]));
+                } else {
+                    const float z = acc[pb][cb][r] + bias[oc];
+                    const bf16 zb = (bf16)z;
+                    if (zout) zout[o] = zb;
+                    if (MODE == MODE_FWD_GELU) aout[o] = (bf16)gelu((float)zb);
+                }
+            }
 }
 
 // Partial weight gradient of a 64-output layer over a strip of row segments:
@@ -350,6 +465,7 @@ struct CnnImpl {
     cnn::bf16 *z[5] = {nullptr}, *a[5] = {nullptr};     // pre-activations and activations
     cnn::bf16 *dz0 = nullptr, *dz1 = nullptr;           // gradient ping-pong [B][H][W][64]
     float *out = nullptr, *gout = nullptr, *partial = nullptr, *loss_part = nullptr;
+    bool direct_conv = false;  // NSE2D_CNN_DIRECT=1: the unstaged convolution kernel
     size_t partial_floats = 0;
     int wgrad_blocks = 0;
 };
@@ -372,19 +488,26 @@ void refresh_bf16(CnnImpl* c) {
 template <int CIN, int MODE>
 void launch_conv(CnnImpl* c, const cnn::bf16* in, const cnn::bf16* w, const float* bias, cnn::bf16* zo, cnn::bf16* ao,
                  const cnn::bf16* zp, int B) {
-    const unsigned blocks = (unsigned)B * c->H * (c->W / 64);
-    cnn::conv_mfma_kernel<CIN, MODE><<<blocks, 256, 0, c->s>>>(in, w, bias, zo, ao, zp, c->H, c->W);
+    if (c->direct_conv) {  // the unstaged kernel (kept as a cross-check of the staged one)
+        const unsigned blocks = (unsigned)B * c->H * (c->W / 64);
+        cnn::conv_mfma_kernel<CIN, MODE><<<blocks, 256, 0, c->s>>>(in, w, bias, zo, ao, zp, c->H, c->W);
+    } else {
+        const unsigned blocks = (unsigned)B * ceil_div(c->H, cnn::kTileRows) * (c->W / 64);
+        cnn::conv_lds_kernel<CIN, MODE><<<blocks, 256, 0, c->s>>>(in, w, bias, zo, ao, zp, c->H, c->W);
+    }
     NSE2D_LAUNCH_CHECK();
 }
 
-void forward(CnnImpl* c, const float* d_in, int B) {
+// keep = false (inference): the pre-activations z are not stored, so no backward may follow.
+void forward(CnnImpl* c, const float* d_in, int B, bool keep = true) {
     const size_t npix = (size_t)B * c->H * c->W;
     cnn::pack_input_kernel<<<ceil_div(npix * cnn::kCin0, 256), 256, 0, c->s>>>(d_in, c->x0, npix);
     NSE2D_LAUNCH_CHECK();
-    launch_conv<cnn::kCin0, cnn::MODE_FWD_GELU>(c, c->x0, c->wb[0], c->p + c->off_b[0], c->z[0], c->a[0], nullptr, B);
+    launch_conv<cnn::kCin0, cnn::MODE_FWD_GELU>(c, c->x0, c->wb[0], c->p + c->off_b[0], keep ? c->z[0] : nullptr,
+                                                c->a[0], nullptr, B);
     for (int l = 1; l < 5; ++l)
-        launch_conv<cnn::kC, cnn::MODE_FWD_GELU>(c, c->a[l - 1], c->wb[l], c->p + c->off_b[l], c->z[l], c->a[l],
-                                                 nullptr, B);
+        launch_conv<cnn::kC, cnn::MODE_FWD_GELU>(c, c->a[l - 1], c->wb[l], c->p + c->off_b[l],
+                                                 keep ? c->z[l] : nullptr, c->a[l], nullptr, B);
     cnn::last_fwd_kernel<<<ceil_div(npix, 256), 256, 0, c->s>>>(c->a[4], c->p + c->off_w[5], c->p + c->off_b[5],
                                                                c->out, c->H, c->W, B);
     NSE2D_LAUNCH_CHECK();
@@ -472,6 +595,7 @@ CnnImpl* cnn_create(int H, int W, int maxB, unsigned long long seed, hipStream_t
     }
     auto* c = new CnnImpl;
     c->H = H; c->W = W; c->maxB = maxB; c->s = s;
+    if (const char* e = std::getenv("NSE2D_CNN_DIRECT")) c->direct_conv = (e[0] == '1');
     size_t o = 0;
     for (int l = 0; l < cnn::kLayers; ++l) {
         const size_t nw = l < 5 ? (size_t)9 * cnn::kC * cinp_of(l) : (size_t)9 * cnn::kC;
@@ -592,6 +716,19 @@ int nse2d_cnn_forward(nse2d_cnn_t c, const float* d_in, float* d_out, int B) {
     if (d_out)
         NSE2D_HIP_CHECK(hipMemcpyAsync(d_out, k->out, sizeof(float) * (size_t)B * k->H * k->W, hipMemcpyDeviceToDevice,
                                        k->s));
+    return 0;
+}
+int nse2d_cnn_infer(nse2d_cnn_t c, const float* d_in, float* d_out, int B) {
+    CnnImpl* k = c->impl;
+    if (B < 1 || B > k->maxB) { set_last_error("nse2d_cnn_infer: batch outside [1, max_batch]"); return -1; }
+    forward(k, d_in, B, false);
+    if (d_out)
+        NSE2D_HIP_CHECK(hipMemcpyAsync(d_out, k->out, sizeof(float) * (size_t)B * k->H * k->W, hipMemcpyDeviceToDevice,
+                                       k->s));
+    return 0;
+}
+int nse2d_cnn_set_direct(nse2d_cnn_t c, int on) {
+    c->impl->direct_conv = on != 0;
     return 0;
 }
 int nse2d_cnn_backward(nse2d_cnn_t c, const float* d_gout, int B) {

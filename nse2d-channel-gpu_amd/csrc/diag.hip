@@ -249,28 +249,6 @@ __global__ void __launch_bounds__(RB) wall_means_kernel(const double* __restrict
     }
 }
 
-// ---- diag.cu:702-816: per-row maxima of |u|, |v|, |v| a over interior rows
-__device__ inline void row_max_uv_body(const double* __restrict__ psi, int Nx, int Ny, double inv_dx,
-                                       double inv_deta, const double* __restrict__ a_node, int j,
-                                       double* __restrict__ partial, double* sm) {
-    double m[3] = {0, 0, 0};
-    if (j > 0 && j < Ny - 1) {
-        const double a = a_node[j];
-        for (int i = threadIdx.x; i < Nx; i += RB) {
-            const int ip = (i + 1 < Nx) ? (i + 1) : 0, im = (i > 0) ? (i - 1) : (Nx - 1);
-            const double dpsideta = (psi[idx2(j + 1, i, Nx)] - psi[idx2(j - 1, i, Nx)]) * (0.5 * inv_deta);
-            const double dpsidx = (psi[idx2(j, ip, Nx)] - psi[idx2(j, im, Nx)]) * (0.5 * inv_dx);
-            const double au = fabs(a * dpsideta), av = fabs(-dpsidx);
-            m[0] = fmax(m[0], au);
-            m[1] = fmax(m[1], av);
-            m[2] = fmax(m[2], av * a);
-        }
-    }
-    block_max<3>(m, sm);
-    if (threadIdx.x == 0)
-        for (int a = 0; a < 3; ++a) partial[(size_t)a * Ny + j] = m[a];
-}
-
 __global__ void __launch_bounds__(RB) row_max_uv_kernel(const double* __restrict__ psi, int Nx, int Ny,
                                                         double inv_dx, double inv_deta,
                                                         const double* __restrict__ a_node,

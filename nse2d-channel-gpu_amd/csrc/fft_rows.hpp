@@ -201,9 +201,10 @@ __device__ __forceinline__ void forward_row(const double* __restrict__ src_row, 
 // Forward: real rows of omega ([rows][2M]) -> packed half spectra ([rows][M]).
 template <int M, int R>
 __global__ void __launch_bounds__(M / R) rows_forward(const double* __restrict__ omega, cplx* __restrict__ hat,
-                                                      const cplx* __restrict__ twp, const cplx* __restrict__ tw) {
+                                                      const cplx* __restrict__ twp, const cplx* __restrict__ tw,
+                                                      int row0) {
     __shared__ cplx lds[Plan<M, R>::LDS];
-    const size_t row = blockIdx.x;
+    const size_t row = (size_t)blockIdx.x + row0;
     forward_row<M, R>(omega + row * 2 * M, hat + row * M, lds, twp, tw);
 }
 
@@ -253,6 +254,11 @@ struct WallClosure {
     const double* dy_edge;
     double psi_bot, psi_top;
     int Ny, free_slip;
+    // nonzero: the launch covers the interior rows only (Ny - 2 workgroups) and the workgroups of
+    // rows 1 and Ny-2 also write psi's wall rows, psi_bot and psi_top. The transform of a wall row
+    // of the solved spectrum, (Nx psi_wall, 0, ..., 0), is that constant exactly (Nx a power of
+    // two), so the result is the same as transforming the wall rows.
+    int fill_walls;
 };
 
 // Inverse of workgroup b of nrow (one field): packed half spectra -> real rows, scaled by `scale`
@@ -264,7 +270,8 @@ __device__ __forceinline__ void inverse_block(const cplx* __restrict__ hat, doub
                                      const cplx* __restrict__ twp, const cplx* __restrict__ tw, double scale,
                                      const WallClosure& wc, int b, int nrow) {
     using L = LastPass<M, R>;
-    const int row = (b == 0) ? nrow - 2 : (b < nrow - 1 ? b - 1 : b);
+    // interior launch: b = 0 -> Ny-2, b in [1, Ny-3] -> b
+    const int row = wc.fill_walls ? (b == 0 ? wc.Ny - 2 : b) : ((b == 0) ? nrow - 2 : (b < nrow - 1 ? b - 1 : b));
     cplx v[R];
     inverse_row<M, R>(hat + (size_t)row * M, v, lds, twp, tw, scale);
     cplx* dst = reinterpret_cast<cplx*>(psi + (size_t)row * 2 * M);
@@ -272,6 +279,15 @@ __device__ __forceinline__ void inverse_block(const cplx* __restrict__ hat, doub
     for (int i = 0; i < L::NI; ++i)
 #pragma unroll
         for (int q = 0; q < L::Rp; ++q) dst[L::pos(i, q)] = v[i * L::Rp + q];
+    if (wc.fill_walls && (row == 1 || row == wc.Ny - 2)) {
+        const bool bot = (row == 1);
+        const double pw = bot ? wc.psi_bot : wc.psi_top;
+        cplx* wall = reinterpret_cast<cplx*>(psi + (size_t)(bot ? 0 : wc.Ny - 1) * 2 * M);
+#pragma unroll
+        for (int i = 0; i < L::NI; ++i)
+#pragma unroll
+            for (int q = 0; q < L::Rp; ++q) wall[L::pos(i, q)] = {pw, pw};
+    }
     if (!wc.omega || (row != 1 && row != wc.Ny - 2)) return;  // uniform over the workgroup
     const bool bottom = (row == 1);
     cplx* wrow = reinterpret_cast<cplx*>(wc.omega + (size_t)(bottom ? 0 : wc.Ny - 1) * 2 * M);

@@ -265,7 +265,7 @@ void closure_hook(void* ctx, const double* w, const double* psi, float* dst, hip
     LesImpl* L = static_cast<LesImpl*>(ctx);
     (void)s;  // the channel and L share one stream (nse2d_les_attach)
     features(L, w, psi, L->Xb, L->norm);
-    nse2d_cnn_forward(L->cnn, L->Xb, L->out, 1);
+    nse2d_cnn_infer(L->cnn, L->Xb, L->out, 1);
     const Grid& g = L->c;
     closure_scale_kernel<<<dim3(ceil_div(g.d.Nx, 256), g.d.Ny), 256, 0, L->s>>>(
         L->out, g.d.Nx, g.d.Ny, (float)L->mean[kFeat], (float)L->stdv[kFeat], dst);

@@ -111,6 +111,30 @@ __device__ __forceinline__ void thom_values(const double* __restrict__ psi, int 
     }
 }
 
+// diag.cu:702-816: maxima of |u|, |v| and |v| a over interior row j (rows 0 and Ny-1 give 0), by one
+// workgroup (blockDim.x a multiple of 64); partial[a * Ny + j] receives maximum a. `sm` needs 48 doubles.
+__device__ __forceinline__ void row_max_uv_body(const double* __restrict__ psi, int Nx, int Ny, double inv_dx,
+                                       double inv_deta, const double* __restrict__ a_node, int j,
+                                       double* __restrict__ partial, double* sm) {
+    double m[3] = {0, 0, 0};
+    if (j > 0 && j < Ny - 1) {
+        const double a = a_node[j];
+        for (int i = threadIdx.x; i < Nx; i += blockDim.x) {
+            const int ip = (i + 1 < Nx) ? (i + 1) : 0, im = (i > 0) ? (i - 1) : (Nx - 1);
+            const double dpsideta = (psi[idx2(j + 1, i, Nx)] - psi[idx2(j - 1, i, Nx)]) * (0.5 * inv_deta);
+            const double dpsidx = (psi[idx2(j, ip, Nx)] - psi[idx2(j, im, Nx)]) * (0.5 * inv_dx);
+            const double au = fabs(a * dpsideta), av = fabs(-dpsidx);
+            m[0] = fmax(m[0], au);
+            m[1] = fmax(m[1], av);
+            m[2] = fmax(m[2], av * a);
+        }
+    }
+    block_max<3>(m, sm);
+    if (threadIdx.x == 0)
+        for (int a = 0; a < 3; ++a) partial[(size_t)a * Ny + j] = m[a];
+}
+
+
 void launch_thom(double* w, const double* psi, int Nx, int Ny, const double* dy_edge,
                  double psi_bot, double psi_top, int free_slip, hipStream_t s);
 

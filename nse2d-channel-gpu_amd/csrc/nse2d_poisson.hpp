@@ -20,10 +20,13 @@ void poisson_solve(PoissonImpl* p, const double* omega, double* psi, double psi_
 void poisson_destroy(PoissonImpl* p);
 // The three phases of poisson_solve, for callers that time them separately.
 void poisson_forward(PoissonImpl* p, const double* omega, hipStream_t s);
+// The forward phase over the interior rows only (the solve never reads the wall rows' spectrum).
+void poisson_forward_interior(PoissonImpl* p, const double* omega, hipStream_t s);
 void poisson_tridiag(PoissonImpl* p, double psi_bot, double psi_top, hipStream_t s);
 void poisson_inverse(PoissonImpl* p, double* psi, hipStream_t s);
 // The inverse phase with the Thom closure of omega's wall rows fused in (ops.cu:113-151, or
-// omega_wall = 0 with free_slip). Returns false, having launched nothing, when the plan's row FFT
+// omega_wall = 0 with free_slip), launched over the interior rows; psi's wall rows get psi_bot and
+// psi_top, which is what their transform gives. Returns false, having launched nothing, when the plan's row FFT
 // cannot fuse it (the small-row LDS kernels); the caller then runs poisson_inverse and the
 // closure kernel.
 bool poisson_fuses_closure(const PoissonImpl* p);

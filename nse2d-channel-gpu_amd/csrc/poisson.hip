@@ -373,7 +373,9 @@ __global__ void __launch_bounds__(T) thomas_chunked_kernel(cplx* __restrict__ ha
                                                            const double* __restrict__ gny, double bc_bot,
                                                            double bc_top) {
     __shared__ ThomasShared<RC, KC, T> S;
-    thomas_group<RC, KC, T>(S, blockIdx.x, hat, M, Ni, a_sub, c_sup, kx2, cpb, gny, bc_bot, bc_top);
+    // column groups contiguous per XCD: groups narrower than a 128-byte line share lines in one L2
+    thomas_group<RC, KC, T>(S, xcd_tile(blockIdx.x, gridDim.x), hat, M, Ni, a_sub, c_sup, kx2, cpb, gny, bc_bot,
+                            bc_top);
 }
 
 // A group of equally sized spectra sharing one plan: workgroup b solves column group
@@ -387,9 +389,9 @@ __global__ void __launch_bounds__(T) thomas_chunked_group_kernel(cplx* const* __
                                                                  const double* __restrict__ gny, double bc_bot,
                                                                  double bc_top) {
     __shared__ ThomasShared<RC, KC, T> S;
-    const int m = blockIdx.x / groups;
-    thomas_group<RC, KC, T>(S, blockIdx.x - m * groups, hats[m], M, Ni, a_sub, c_sup, kx2, cpb, gny, bc_bot,
-                            bc_top);
+    const int b = xcd_tile(blockIdx.x, gridDim.x);
+    const int m = b / groups;
+    thomas_group<RC, KC, T>(S, b - m * groups, hats[m], M, Ni, a_sub, c_sup, kx2, cpb, gny, bc_bot, bc_top);
 }
 
 int ilog2(int n) {
@@ -461,9 +463,10 @@ std::vector<cplx> fft_pass_twiddles(int M, int R) {
     return t;
 }
 template <int M, int R>
-void launch_rows_forward(const double* omega, cplx* hat, int rows, const cplx* twp, const cplx* tw, hipStream_t s) {
+void launch_rows_forward(const double* omega, cplx* hat, int rows, const cplx* twp, const cplx* tw, hipStream_t s,
+                         int row0 = 0) {
     static_assert(fft::Plan<M, R>::T >= 64, "row FFT needs at least one wavefront");
-    fft::rows_forward<M, R><<<rows, M / R, 0, s>>>(omega, hat, twp, tw);
+    fft::rows_forward<M, R><<<rows, M / R, 0, s>>>(omega, hat, twp, tw, row0);
 }
 template <int M, int R>
 void launch_rows_inverse(const cplx* hat, double* psi, int rows, const cplx* twp, const cplx* tw, double scale,
@@ -613,6 +616,20 @@ void poisson_forward(PoissonImpl* p, const double* omega, hipStream_t s) {
         fft_rows_forward_kernel<2><<<p->Ny, p->threads, p->lds_bytes, s>>>(omega, p->d_hat, p->Nx, p->log2M, p->d_tw);
     NSE2D_LAUNCH_CHECK();
 }
+void poisson_forward_interior(PoissonImpl* p, const double* omega, hipStream_t s) {
+    if (!p->fft_radix) {
+        poisson_forward(p, omega, s);
+        return;
+    }
+    switch (p->M * 100 + p->fft_radix) {
+#define NSE2D_FWDI_CASE(M, R) \
+    case M * 100 + R: launch_rows_forward<M, R>(omega, p->d_hat, p->Ny - 2, p->d_twp, p->d_tw, s, 1); break;
+        NSE2D_FFT_PLANS(NSE2D_FWDI_CASE)
+#undef NSE2D_FWDI_CASE
+        default: std::abort();
+    }
+    NSE2D_LAUNCH_CHECK();
+}
 void poisson_tridiag(PoissonImpl* p, double psi_bot, double psi_top, hipStream_t s) {
     const int Ni = p->Ny - 2;
     const double bb = (double)p->Nx * psi_bot, bt = (double)p->Nx * psi_top;
@@ -642,10 +659,10 @@ bool poisson_inverse_closure(PoissonImpl* p, double* psi, double* omega, const d
                              double psi_top, int free_slip, hipStream_t s) {
     if (!p->fft_radix) return false;
     const double scale = (double)(1.0 / (double)p->Nx);
-    const fft::WallClosure wc{omega, dy_edge, psi_bot, psi_top, p->Ny, free_slip};
+    const fft::WallClosure wc{omega, dy_edge, psi_bot, psi_top, p->Ny, free_slip, 1};
     switch (p->M * 100 + p->fft_radix) {
 #define NSE2D_INVC_CASE(M, R) \
-    case M * 100 + R: launch_rows_inverse<M, R>(p->d_hat, psi, p->Ny, p->d_twp, p->d_tw, scale, wc, s); break;
+    case M * 100 + R: launch_rows_inverse<M, R>(p->d_hat, psi, p->Ny - 2, p->d_twp, p->d_tw, scale, wc, s); break;
         NSE2D_FFT_PLANS(NSE2D_INVC_CASE)
 #undef NSE2D_INVC_CASE
         default: std::abort();

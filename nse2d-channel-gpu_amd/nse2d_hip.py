@@ -91,6 +91,8 @@ SIGNATURES = {
     "nse2d_cnn_get_grads": (None, [_vp, _vp]),
     "nse2d_cnn_forward": (_i, [_vp, _vp, _vp, _i]),
     "nse2d_cnn_backward": (_i, [_vp, _vp, _i]),
+    "nse2d_cnn_infer": (_i, [_vp, _vp, _vp, _i]),
+    "nse2d_cnn_set_direct": (_i, [_vp, _i]),
     "nse2d_cnn_train_step": (_d, [_vp, _vp, _vp, _i, ctypes.c_float]),
     "nse2d_cnn_adam_step": (_i, [_vp, ctypes.c_float]),
     "nse2d_cnn_eval_loss": (_d, [_vp, _vp, _vp, _i]),

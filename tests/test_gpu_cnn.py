@@ -100,3 +100,31 @@ def test_training_reduces_loss_on_a_fixed_batch():
     assert abs(losses[0] - co.mse(out, tgt)[0]) <= 1e-2 * losses[0]
     assert losses[-1] < 0.2 * losses[0]
     nh.lib().nse2d_cnn_destroy(c)
+
+
+@pytest.mark.parametrize("H,W,B", [(9, 64, 2), (7, 128, 1), (33, 64, 3)])
+def test_staged_convolution_matches_direct(H, W, B):
+    """The LDS-staged convolution accumulates the same products in the same order as the kernel
+    that reads its operands from global memory (taps beyond the walls add exact zeros), so the
+    network outputs and gradients agree bit for bit; inference (no pre-activations kept) gives
+    the same outputs."""
+    print(f"[cnn] staged vs direct {B}x{H}x{W}", flush=True)
+    c, p = make(H, W, B, seed=9)
+    rng = np.random.default_rng(H + W)
+    inp = rng.normal(size=(B, H, W, co.NIN)).astype(np.float32)
+    gout = rng.normal(size=(B, H, W)).astype(np.float32)
+    d_in, d_out, d_g = nh.DeviceArray(inp), nh.DeviceArray(shape=(B, H, W)), nh.DeviceArray(gout)
+    res = {}
+    for direct in (1, 0):
+        assert nh.lib().nse2d_cnn_set_direct(c, direct) == 0
+        assert nh.lib().nse2d_cnn_forward(c, d_in.ptr, d_out.ptr, B) == 0
+        out = d_out.download()
+        assert nh.lib().nse2d_cnn_backward(c, d_g.ptr, B) == 0
+        g = np.zeros(p.size, np.float32)
+        nh.lib().nse2d_cnn_get_grads(c, g.ctypes.data)
+        assert nh.lib().nse2d_cnn_infer(c, d_in.ptr, d_out.ptr, B) == 0
+        res[direct] = (out, g, d_out.download())
+    nh.lib().nse2d_cnn_destroy(c)
+    assert np.array_equal(res[0][0], res[1][0])
+    assert np.array_equal(res[0][1], res[1][1])
+    assert np.array_equal(res[0][2], res[0][0]) and np.array_equal(res[1][2], res[1][0])

@@ -233,7 +233,8 @@ def test_step_stagewise(Nx, Ny, mms, fs):
     _check_stagewise(errs)
 
 
-STAGE_TILES = ["0"] + [f"{ry}x{sy}x{pf}" for ry, sy in ((8, 4), (16, 4), (16, 2), (32, 1)) for pf in (1, 2, 3)]
+STAGE_TILES = (["0"] + [f"{ry}x{sy}x{pf}" for ry, sy in ((8, 4), (16, 4), (16, 2), (32, 1)) for pf in (1, 2, 3)]
+               + ["w8x4x1", "w8x4x2", "w4x4x1", "w16x2x1", "w8x2x1"])  # two columns per lane (Nx % 128 == 0)
 
 
 @pytest.mark.parametrize("tile", STAGE_TILES)
