@@ -27,8 +27,16 @@ namespace fft {
 struct cplx { double x, y; };
 __device__ __forceinline__ cplx cadd(cplx a, cplx b) { return {a.x + b.x, a.y + b.y}; }
 __device__ __forceinline__ cplx csub(cplx a, cplx b) { return {a.x - b.x, a.y - b.y}; }
-__device__ __forceinline__ cplx cmul(cplx a, cplx b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
-__device__ __forceinline__ cplx cmulc(cplx a, cplx b) { return {a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y}; }  // a conj(b)
+// Complex products with the fused multiply-adds written out: left to the compiler, a*b - c*d may
+// be fused on either product depending on the surrounding code, so two kernels that share these
+// functions (one row or two rows per workgroup, single channel or ensemble) could differ in the
+// last bit.
+__device__ __forceinline__ cplx cmul(cplx a, cplx b) {
+    return {__builtin_fma(a.x, b.x, -(a.y * b.y)), __builtin_fma(a.x, b.y, a.y * b.x)};
+}
+__device__ __forceinline__ cplx cmulc(cplx a, cplx b) {  // a conj(b)
+    return {__builtin_fma(a.x, b.x, a.y * b.y), __builtin_fma(a.y, b.x, -(a.x * b.y))};
+}
 __device__ __forceinline__ cplx cconj(cplx a) { return {a.x, -a.y}; }
 __device__ __forceinline__ cplx cscale(cplx a, double s) { return {a.x * s, a.y * s}; }
 
@@ -78,7 +86,8 @@ __device__ __forceinline__ cplx twiddle_const(cplx a, int e) {
     if (m == 12) return mul_mi<!INV>(a);
     const double c = cs[m], s = sn[m];
     // forward: a (c - i s); inverse: a (c + i s)
-    return INV ? cplx{a.x * c - a.y * s, a.y * c + a.x * s} : cplx{a.x * c + a.y * s, a.y * c - a.x * s};
+    return INV ? cplx{__builtin_fma(a.x, c, -(a.y * s)), __builtin_fma(a.y, c, a.x * s)}
+               : cplx{__builtin_fma(a.x, c, a.y * s), __builtin_fma(a.y, c, -(a.x * s))};
 }
 
 // In-register DFT of N = 1, 2, 4, 8 or 16 values x[0], x[S], .., x[(N-1)S], natural order in and
@@ -167,17 +176,21 @@ struct LastPass {
     __device__ static int pos(int i, int q) { return (int)threadIdx.x + i * P::T + q * Ns; }
 };
 
-// One forward row: Nx = 2M reals -> packed half spectrum (M complex). Uses all of lds.
+// The R complex values z_{tid + q T} of one real row (the first pass's inputs).
 template <int M, int R>
-__device__ __forceinline__ void forward_row(const double* __restrict__ src_row, cplx* __restrict__ dst,
-                                   cplx* lds, const cplx* __restrict__ twp, const cplx* __restrict__ tw) {
+__device__ __forceinline__ void load_row(const double* __restrict__ src_row, cplx (&v)[R]) {
+    const cplx* src = reinterpret_cast<const cplx*>(src_row);
+#pragma unroll
+    for (int q = 0; q < R; ++q) v[q] = src[(int)threadIdx.x + q * Plan<M, R>::T];
+}
+
+// One forward row from its loaded inputs: -> packed half spectrum (M complex). Uses all of lds.
+template <int M, int R>
+__device__ __forceinline__ void forward_row_from(cplx (&v)[R], cplx* __restrict__ dst, cplx* lds,
+                                                 const cplx* __restrict__ twp, const cplx* __restrict__ tw) {
     using P = Plan<M, R>;
     constexpr int T = P::T;
     const int tid = threadIdx.x;
-    const cplx* src = reinterpret_cast<const cplx*>(src_row);
-    cplx v[R];
-#pragma unroll
-    for (int q = 0; q < R; ++q) v[q] = src[tid + q * T];
     dft<R, false>(v);  // pass 0: Ns = 1, no twiddles
 #pragma unroll
     for (int q = 0; q < R; ++q) lds[lds_pad(tid * R + q)] = v[q];
@@ -198,6 +211,15 @@ __device__ __forceinline__ void forward_row(const double* __restrict__ src_row, 
     }
 }
 
+// One forward row: Nx = 2M reals -> packed half spectrum (M complex). Uses all of lds.
+template <int M, int R>
+__device__ __forceinline__ void forward_row(const double* __restrict__ src_row, cplx* __restrict__ dst,
+                                   cplx* lds, const cplx* __restrict__ twp, const cplx* __restrict__ tw) {
+    cplx v[R];
+    load_row<M, R>(src_row, v);
+    forward_row_from<M, R>(v, dst, lds, twp, tw);
+}
+
 // Forward: real rows of omega ([rows][2M]) -> packed half spectra ([rows][M]).
 template <int M, int R>
 __global__ void __launch_bounds__(M / R) rows_forward(const double* __restrict__ omega, cplx* __restrict__ hat,
@@ -206,6 +228,26 @@ __global__ void __launch_bounds__(M / R) rows_forward(const double* __restrict__
     __shared__ cplx lds[Plan<M, R>::LDS];
     const size_t row = (size_t)blockIdx.x + row0;
     forward_row<M, R>(omega + row * 2 * M, hat + row * M, lds, twp, tw);
+}
+
+// Two rows per workgroup, rows row0 + b and row0 + b + gridDim.x (when < row0 + nrows): the
+// second row's loads are issued before the first row's LDS passes, so that they stream while the
+// first row is transformed and the chip does not alternate between loading and computing.
+template <int M, int R>
+__global__ void __launch_bounds__(M / R) rows_forward2(const double* __restrict__ omega, cplx* __restrict__ hat,
+                                                       const cplx* __restrict__ twp, const cplx* __restrict__ tw,
+                                                       int row0, int nrows) {
+    __shared__ cplx lds[Plan<M, R>::LDS];
+    const int ra = blockIdx.x, rb = blockIdx.x + gridDim.x;
+    const bool two = rb < nrows;
+    cplx va[R], vb[R];
+    load_row<M, R>(omega + (size_t)(row0 + ra) * 2 * M, va);
+    if (two) load_row<M, R>(omega + (size_t)(row0 + rb) * 2 * M, vb);
+    forward_row_from<M, R>(va, hat + (size_t)(row0 + ra) * M, lds, twp, tw);
+    if (two) {
+        __syncthreads();  // lds is reused
+        forward_row_from<M, R>(vb, hat + (size_t)(row0 + rb) * M, lds, twp, tw);
+    }
 }
 
 // The same for a group of equally sized fields (ensemble members): workgroup b transforms row
