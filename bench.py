@@ -217,7 +217,7 @@ def run_les(a):
     r = lx.run(fine, coarse, spinup_time=a.les_spinup, window_time=a.les_window, n_pairs=a.les_pairs,
                holdout=max(1, a.les_pairs // 20), train_steps=a.les_train_steps, batch=32, lr=1e-3,
                n_samples=min(100, a.les_pairs), log=lambda m: print("les: " + m, file=sys.stderr, flush=True))
-    cl, nc = r["runs"]["closure"], r["runs"]["no_closure"]
+    cl, nc, cs = r["runs"]["closure"], r["runs"]["no_closure"], r["runs"]["closure_per_step"]
     out = {
         "metric": "LES wall-clock speedup over DNS per unit simulated time (512x257 closure LES vs 4096x2049 DNS)",
         "value": r["speedup_vs_dns"], "unit": "x", "n_gpus": 1, "higher_is_better": True, "vs_baseline": None,
@@ -227,6 +227,8 @@ def run_les(a):
                    "pairs": a.les_pairs, "batch": 32, "train_steps": a.les_train_steps,
                    "spinup_time": a.les_spinup, "window_time": a.les_window},
         "re_tau_err_closure": cl.get("re_tau_rel_err"), "re_tau_err_no_closure": nc.get("re_tau_rel_err"),
+        "speedup_closure_once_per_step": r["speedup_vs_dns_per_step_closure"],
+        "re_tau_err_closure_once_per_step": cs.get("re_tau_rel_err"),
         "U_err_closure": cl.get("U_rel_err"), "U_err_no_closure": nc.get("U_rel_err"),
         "traj_err_closure": cl.get("traj_err_end"), "traj_err_no_closure": nc.get("traj_err_end"),
         "details": r,
@@ -234,11 +236,54 @@ def run_les(a):
     print(json.dumps(out), flush=True)
 
 
+def run_cnn(a):
+    """One line of JSON: the closure network at the LES grid (257x512): inference of one image (the
+    closure term of one coarse state) and one Adam training step at batch 32, with the bf16 MFMA
+    roofline of the convolutions (dense peak 2.5 PFLOP/s)."""
+    L = nh.lib()
+    H, W, B = 257, 512, 32
+    c = L.nse2d_cnn_create(H, W, B, 1, None)
+    if not c:
+        raise RuntimeError(nh.last_error())
+    rng = np.random.default_rng(0)
+    x = nh.DeviceArray(rng.normal(size=(B, H, W, 4)).astype(np.float32))
+    y = nh.DeviceArray(rng.normal(size=(B, H, W)).astype(np.float32))
+    o = nh.DeviceArray(shape=(B, H, W), dtype=np.float32)
+    # executed convolution FLOPs per pixel: first layer 32 (padded) -> 64, four 64 -> 64, last 64 -> 1
+    flop_px = 2 * 9 * (32 * 64 + 4 * 64 * 64 + 64)
+    for _ in range(a.warmup):
+        L.nse2d_cnn_infer(c, x.ptr, o.ptr, 1)
+    L.nse2d_device_sync()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        L.nse2d_cnn_infer(c, x.ptr, o.ptr, 1)
+    L.nse2d_device_sync()
+    t_inf = (time.perf_counter() - t0) / a.steps
+    L.nse2d_cnn_train_step(c, x.ptr, y.ptr, B, 1e-4)
+    n_train = max(3, a.steps // 10)
+    t0 = time.perf_counter()
+    for _ in range(n_train):
+        L.nse2d_cnn_train_step(c, x.ptr, y.ptr, B, 1e-4)
+    t_train = (time.perf_counter() - t0) / n_train
+    L.nse2d_cnn_destroy(c)
+    inf_tflops = H * W * flop_px / t_inf / 1e12
+    train_tflops = 3 * B * H * W * flop_px / t_train / 1e12
+    out = {"metric": "closure network inference time per coarse state (257x512, batch 1)", "value": t_inf * 1e3,
+           "unit": "ms", "higher_is_better": False, "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+           "vs_baseline": None, "dtype": "bf16 (fp32 accumulate)", "data": "synthetic",
+           "config": {"workload": "6-layer 3x3 CNN, 64 channels, GELU, 257x512 images", "batch_train": B},
+           "inference_tflops": inf_tflops, "train_step_ms": t_train * 1e3, "train_tflops": train_tflops,
+           "roofline": {"bound": "mfma", "achieved": train_tflops, "peak": 2500.0, "unit": "TFLOP/s",
+                        "frac": train_tflops / 2500.0, "kernel": "training step, batch 32"}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["channel", "ensemble", "les"], default="channel",
+    ap.add_argument("--workload", choices=["channel", "ensemble", "les", "cnn"], default="channel",
                     help="channel: the 4096x2049 forced channel (default); ensemble: 64 independent channels; "
-                         "les: learned-closure LES at 512x257 trained on the 4096x2049 DNS")
+                         "les: learned-closure LES at 512x257 trained on the 4096x2049 DNS; cnn: the closure "
+                         "network alone")
     ap.add_argument("--les-spinup", type=float, default=4.0)
     ap.add_argument("--les-window", type=float, default=4.0)
     ap.add_argument("--les-pairs", type=int, default=2000)
@@ -271,6 +316,9 @@ def main():
         return
     if a.workload == "les":
         run_les(a)
+        return
+    if a.workload == "cnn":
+        run_cnn(a)
         return
     c = CFG
     Nx, Ny = c["Nx"], c["Ny"]

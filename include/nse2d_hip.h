@@ -333,8 +333,9 @@ int nse2d_cnn_forward(nse2d_cnn_t c, const float* d_in, float* d_out, int B);
 /* The forward of nse2d_cnn_forward without keeping the pre-activations (inference: no backward
  * may follow). 0 on success. */
 int nse2d_cnn_infer(nse2d_cnn_t c, const float* d_in, float* d_out, int B);
-/* on != 0: convolutions by the kernel that reads its operands straight from global memory
- * instead of the LDS-staged one (for cross-checks; also NSE2D_CNN_DIRECT=1 at creation). */
+/* Convolution kernel selection, for cross-checks (all compute the same products in the same
+ * order): 0 = the persistent LDS-staged kernel (default), 1 = operands straight from global
+ * memory (also NSE2D_CNN_DIRECT=1 at creation), 2 = LDS-staged, one workgroup per row tile. */
 int nse2d_cnn_set_direct(nse2d_cnn_t c, int on);
 /* Gradients of sum(d_gout * out) for the batch of the preceding forward. 0 on success. */
 int nse2d_cnn_backward(nse2d_cnn_t c, const float* d_gout, int B);
@@ -389,8 +390,10 @@ double nse2d_les_train(nse2d_les_t L, int n_train, int steps, int batch, float l
 int nse2d_les_eval(nse2d_les_t L, int first, int count, double* loss_cnn, double* loss_zero);
 /* The closure term of one coarse state into d_out ([Ny_c][Nx_c] fp32, 0 on the wall rows). */
 int nse2d_les_closure(nse2d_les_t L, const double* d_omega_c, const double* d_psi_c, float* d_out);
-/* on != 0: the coarse channel adds the closure of each stage state to its RHS (section 5,
- * nse2d_channel_set_closure); on = 0 detaches it. */
+/* on = 1: the coarse channel adds the closure of each stage state to its RHS (section 5,
+ * nse2d_channel_set_closure); on = 2: the closure of each step's first stage state is evaluated
+ * once and added in all three stages (a third of the network evaluations; the closure term is
+ * then first order in time); on = 0 detaches it. 0 on success. */
 int nse2d_les_attach(nse2d_les_t L, nse2d_channel_t coarse_ch, int on);
 
 /* ------------------------------------------------------------------------------------------

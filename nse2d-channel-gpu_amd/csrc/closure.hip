@@ -20,6 +20,7 @@
 #include "nse2d_common.hpp"
 #include "../../include/nse2d_hip.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -41,15 +42,20 @@ constexpr int kCin0 = 32;   // padded input channels of the first layer
 constexpr int kInputs = 4;  // real input channels
 constexpr int kLayers = 6;
 
+// tanh(u) = 1 - 2 / (1 + e^{2u}) on the native exponential: absolute error ~1e-7, far below the
+// bf16 resolution of the activations it feeds (the GELU below adds it to 1), and a few
+// instructions instead of the library tanhf (the epilogue of every convolution applies it to
+// each output). Saturates correctly: e^{2u} -> inf gives 1, -> 0 gives -1.
+__device__ inline float tanh_fast(float u) { return 1.0f - 2.0f / (1.0f + __expf(2.0f * u)); }
 // GELU, tanh form (Hendrycks and Gimpel): 0.5 z (1 + tanh(s (z + 0.044715 z^3))), s = sqrt(2/pi)
 __device__ inline float gelu(float z) {
     const float s = 0.7978845608028654f;
-    return 0.5f * z * (1.0f + tanhf(s * (z + 0.044715f * z * z * z)));
+    return 0.5f * z * (1.0f + tanh_fast(s * (z + 0.044715f * z * z * z)));
 }
 __device__ inline float gelu_grad(float z) {
     const float s = 0.7978845608028654f;
     const float u = s * (z + 0.044715f * z * z * z);
-    const float th = tanhf(u);
+    const float th = tanh_fast(u);
     return 0.5f * (1.0f + th) + 0.5f * z * (1.0f - th * th) * s * (1.0f + 3.0f * 0.044715f * z * z);
 }
 
@@ -215,6 +221,153 @@ __global__ void __launch_bounds__(256) conv_lds_kernel(const bf16* __restrict__ 
                     if (MODE == MODE_FWD_GELU) aout[o] = (bf16)gelu((float)zb);
                 }
             }
+}
+
+// The persistent form: every weight of the layer (nine taps) is staged in LDS once per workgroup,
+// which then loops over row tiles (tile = blockIdx.x, + gridDim.x, ...). Between tiles the next
+// tile's inputs are fetched into registers while the current tile's outputs are written, so the
+// loop has no per-tap barrier and no exposed global latency inside the MFMA sequence. The
+// products and their order are those of conv_lds_kernel.
+template <int CIN>
+struct ConvLdsAll {
+    static constexpr int CP = CIN + 8;
+    bf16 w[9][kC][CP];
+    bf16 in[kTileRows + 2][66][CP];
+};
+
+// 512 threads: wavefront w computes row y0 + (w & 3), output channels 32 (w >> 2) .. +31, as two
+// 32x32 MFMA tiles (pixels 0..31 and 32..63 of the segment), so that two wavefronts share each
+// SIMD and one's epilogue overlaps the other's MFMAs.
+constexpr int kAllThreads = 512;
+template <int CIN, int MODE>
+__global__ void __launch_bounds__(kAllThreads) conv_lds_all_kernel(const bf16* __restrict__ in,
+                                                                   const bf16* __restrict__ w,
+                                                                   const float* __restrict__ bias,
+                                                                   bf16* __restrict__ zout, bf16* __restrict__ aout,
+                                                                   const bf16* __restrict__ zprev, int H, int W,
+                                                                   int ntiles) {
+    using S = ConvLdsAll<CIN>;
+    constexpr int C8 = CIN / 8;
+    constexpr int NT = kAllThreads;
+    constexpr int NIN = ((kTileRows + 2) * 66 * C8 + NT - 1) / NT;  // input chunks per thread
+    __shared__ alignas(16) S sh;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wrow = wave & 3, cb = wave >> 2;
+    const int nseg = W >> 6, nyt = (H + kTileRows - 1) / kTileRows;
+    const int i = lane & 31, kh = lane >> 5;
+    for (int e = tid; e < 9 * kC * C8; e += NT) {
+        const int t = e / (kC * C8), rem = e % (kC * C8), n = rem / C8, c8 = (rem % C8) * 8;
+        *reinterpret_cast<bf16x8*>(&sh.w[t][n][c8]) = load8(w + ((size_t)t * kC + n) * CIN + c8);
+    }
+    bf16x8 pre[NIN];
+    auto fetch = [&](int tile) {
+        const int seg = tile % nseg, yt = (tile / nseg) % nyt, b = tile / (nseg * nyt);
+        const int x0 = seg << 6, y0 = yt * kTileRows;
+#pragma unroll
+        for (int q = 0; q < NIN; ++q) {
+            const int e = tid + q * NT;
+            if (e < (kTileRows + 2) * 66 * C8) {
+                const int r = e / (66 * C8), rem = e % (66 * C8), px = rem / C8, c8 = (rem % C8) * 8;
+                const int yy = y0 - 1 + r;
+                int xx = x0 - 1 + px;
+                xx = xx < 0 ? xx + W : (xx >= W ? xx - W : xx);
+                if (yy >= 0 && yy < H) pre[q] = load8(in + (((size_t)b * H + yy) * W + xx) * CIN + c8);
+                else
+#pragma unroll
+                    for (int k = 0; k < 8; ++k) pre[q][k] = (bf16)0.0f;
+            }
+        }
+    };
+    auto deposit = [&]() {
+#pragma unroll
+        for (int q = 0; q < NIN; ++q) {
+            const int e = tid + q * NT;
+            if (e < (kTileRows + 2) * 66 * C8) {
+                const int r = e / (66 * C8), rem = e % (66 * C8), px = rem / C8, c8 = (rem % C8) * 8;
+                *reinterpret_cast<bf16x8*>(&sh.in[r][px][c8]) = pre[q];
+            }
+        }
+    };
+    if (blockIdx.x < ntiles) fetch(blockIdx.x);
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        deposit();
+        __syncthreads();
+        floatx16 acc[2];
+        acc[0] = floatx16{0};
+        acc[1] = floatx16{0};
+#pragma unroll
+        for (int t = 0; t < 9; ++t) {
+            int dy = t / 3 - 1, dx = t % 3 - 1;
+            if (MODE == MODE_BWD_DATA) { dy = -dy; dx = -dx; }
+            const int lr = wrow + 1 + dy;
+#pragma unroll
+            for (int kb = 0; kb < CIN / 16; ++kb) {
+                const int co = kb * 16 + kh * 8;
+                const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(&sh.in[lr][i + 1 + dx][co]);
+                const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(&sh.in[lr][i + 33 + dx][co]);
+                const bf16x8 bw = *reinterpret_cast<const bf16x8*>(&sh.w[t][(cb << 5) + i][co]);
+                acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, bw, acc[0], 0, 0, 0);
+                acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, bw, acc[1], 0, 0, 0);
+            }
+        }
+        const int nxt = tile + gridDim.x;
+        if (nxt < ntiles) fetch(nxt);  // in flight during the stores below
+        const int seg = tile % nseg, yt = (tile / nseg) % nyt, b = tile / (nseg * nyt);
+        const int x0 = seg << 6, y = yt * kTileRows + wrow;
+        if (y < H) {
+#pragma unroll
+            for (int pb = 0; pb < 2; ++pb)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int px = x0 + (pb << 5) + ((r >> 2) << 3) + (kh << 2) + (r & 3);
+                    const int oc = (cb << 5) + i;
+                    const size_t o = (((size_t)b * H + y) * W + px) * kC + oc;
+                    if (MODE == MODE_BWD_DATA) {
+                        aout[o] = (bf16)(acc[pb][r] * gelu_grad((float)zprev[o]));
+                    } else {
+                        const float z = acc[pb][r] + bias[oc];
+                        const bf16 zb = (bf16)z;
+                        if (zout) zout[o] = zb;
+                        if (MODE == MODE_FWD_GELU) aout[o] = (bf16)gelu((float)zb);
+                    }
+                }
+        }
+        __syncthreads();  // every wavefront is done with sh.in before the next deposit
+    }
+}
+
+// Last layer (64 -> 1) with eight lanes per pixel: lane octet s of a pixel reads channels
+// 8s..8s+7 of each tap (a wavefront load covers 8 whole pixels, 1 KB contiguous); the octets'
+// partial sums are added with two lane exchanges.
+__global__ void __launch_bounds__(256) last_fwd_octet_kernel(const bf16* __restrict__ a, const float* __restrict__ w,
+                                                             const float* __restrict__ bias, float* __restrict__ out,
+                                                             int H, int W, int B) {
+    const int sub = threadIdx.x & 7;
+    const size_t p = (size_t)blockIdx.x * 32 + (threadIdx.x >> 3);
+    const size_t npix = (size_t)B * H * W;
+    float wr[9][8];
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) wr[t][q] = w[t * kC + sub * 8 + q];
+    float s = 0.0f;
+    if (p < npix) {
+        const int x = (int)(p % W), y = (int)((p / W) % H), b = (int)(p / ((size_t)W * H));
+#pragma unroll
+        for (int t = 0; t < 9; ++t) {
+            const int yy = y + t / 3 - 1;
+            if (yy < 0 || yy >= H) continue;
+            int xx = x + t % 3 - 1;
+            xx = xx < 0 ? xx + W : (xx >= W ? xx - W : xx);
+            const bf16x8 v = load8(a + (((size_t)b * H + yy) * W + xx) * kC + sub * 8);
+#pragma unroll
+            for (int q = 0; q < 8; ++q) s += (float)v[q] * wr[t][q];
+        }
+    }
+    s += __shfl_xor(s, 1, 64);
+    s += __shfl_xor(s, 2, 64);
+    s += __shfl_xor(s, 4, 64);
+    if (sub == 0 && p < npix) out[p] = s + bias[0];
 }
 
 // Partial weight gradient of a 64-output layer over a strip of row segments:
@@ -466,6 +619,8 @@ struct CnnImpl {
     cnn::bf16 *dz0 = nullptr, *dz1 = nullptr;           // gradient ping-pong [B][H][W][64]
     float *out = nullptr, *gout = nullptr, *partial = nullptr, *loss_part = nullptr;
     bool direct_conv = false;  // NSE2D_CNN_DIRECT=1: the unstaged convolution kernel
+    bool persistent_conv = true;  // NSE2D_CNN_PERSISTENT=0: the per-tile staged kernel
+    int cus = 256;                // compute units of the device (grid of the persistent kernel)
     size_t partial_floats = 0;
     int wgrad_blocks = 0;
 };
@@ -488,12 +643,18 @@ void refresh_bf16(CnnImpl* c) {
 template <int CIN, int MODE>
 void launch_conv(CnnImpl* c, const cnn::bf16* in, const cnn::bf16* w, const float* bias, cnn::bf16* zo, cnn::bf16* ao,
                  const cnn::bf16* zp, int B) {
-    if (c->direct_conv) {  // the unstaged kernel (kept as a cross-check of the staged one)
+    const unsigned tiles = (unsigned)B * ceil_div(c->H, cnn::kTileRows) * (c->W / 64);
+    if (c->direct_conv) {  // the unstaged kernel (kept as a cross-check of the staged ones)
         const unsigned blocks = (unsigned)B * c->H * (c->W / 64);
         cnn::conv_mfma_kernel<CIN, MODE><<<blocks, 256, 0, c->s>>>(in, w, bias, zo, ao, zp, c->H, c->W);
+    } else if (c->persistent_conv) {
+        // one workgroup per CU for the 64-channel layers (137 KB of LDS), two for the first layer
+        const unsigned per_cu = CIN == cnn::kC ? 1 : 2;
+        const unsigned blocks = std::min<unsigned>(tiles, per_cu * (unsigned)c->cus);
+        cnn::conv_lds_all_kernel<CIN, MODE><<<blocks, cnn::kAllThreads, 0, c->s>>>(in, w, bias, zo, ao, zp, c->H,
+                                                                                  c->W, (int)tiles);
     } else {
-        const unsigned blocks = (unsigned)B * ceil_div(c->H, cnn::kTileRows) * (c->W / 64);
-        cnn::conv_lds_kernel<CIN, MODE><<<blocks, 256, 0, c->s>>>(in, w, bias, zo, ao, zp, c->H, c->W);
+        cnn::conv_lds_kernel<CIN, MODE><<<tiles, 256, 0, c->s>>>(in, w, bias, zo, ao, zp, c->H, c->W);
     }
     NSE2D_LAUNCH_CHECK();
 }
@@ -508,8 +669,8 @@ void forward(CnnImpl* c, const float* d_in, int B, bool keep = true) {
     for (int l = 1; l < 5; ++l)
         launch_conv<cnn::kC, cnn::MODE_FWD_GELU>(c, c->a[l - 1], c->wb[l], c->p + c->off_b[l],
                                                  keep ? c->z[l] : nullptr, c->a[l], nullptr, B);
-    cnn::last_fwd_kernel<<<ceil_div(npix, 256), 256, 0, c->s>>>(c->a[4], c->p + c->off_w[5], c->p + c->off_b[5],
-                                                               c->out, c->H, c->W, B);
+    cnn::last_fwd_octet_kernel<<<ceil_div(npix, 32), 256, 0, c->s>>>(c->a[4], c->p + c->off_w[5],
+                                                                    c->p + c->off_b[5], c->out, c->H, c->W, B);
     NSE2D_LAUNCH_CHECK();
 }
 
@@ -596,6 +757,13 @@ CnnImpl* cnn_create(int H, int W, int maxB, unsigned long long seed, hipStream_t
     auto* c = new CnnImpl;
     c->H = H; c->W = W; c->maxB = maxB; c->s = s;
     if (const char* e = std::getenv("NSE2D_CNN_DIRECT")) c->direct_conv = (e[0] == '1');
+    if (const char* e = std::getenv("NSE2D_CNN_PERSISTENT")) c->persistent_conv = (e[0] != '0');
+    {
+        int dev = 0;
+        NSE2D_HIP_CHECK(hipGetDevice(&dev));
+        NSE2D_HIP_CHECK(hipDeviceGetAttribute(&c->cus, hipDeviceAttributeMultiprocessorCount, dev));
+        if (c->cus < 1) c->cus = 1;
+    }
     size_t o = 0;
     for (int l = 0; l < cnn::kLayers; ++l) {
         const size_t nw = l < 5 ? (size_t)9 * cnn::kC * cinp_of(l) : (size_t)9 * cnn::kC;
@@ -728,7 +896,8 @@ int nse2d_cnn_infer(nse2d_cnn_t c, const float* d_in, float* d_out, int B) {
     return 0;
 }
 int nse2d_cnn_set_direct(nse2d_cnn_t c, int on) {
-    c->impl->direct_conv = on != 0;
+    c->impl->direct_conv = on == 1;
+    c->impl->persistent_conv = on != 2;
     return 0;
 }
 int nse2d_cnn_backward(nse2d_cnn_t c, const float* d_gout, int B) {

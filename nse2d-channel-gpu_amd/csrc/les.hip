@@ -213,6 +213,8 @@ struct LesImpl {
     Norm norm{};
     double mean[kFeat + 1] = {0}, stdv[kFeat + 1] = {1, 1, 1, 1, 1};
     nse2d_cnn_t cnn = nullptr;
+    int attach_mode = 1;   // 1: closure of every stage state; 2: of the step's first stage, held
+    long long hook_calls = 0;
 };
 
 namespace {
@@ -264,6 +266,9 @@ Norm raw_norm() {
 void closure_hook(void* ctx, const double* w, const double* psi, float* dst, hipStream_t s) {
     LesImpl* L = static_cast<LesImpl*>(ctx);
     (void)s;  // the channel and L share one stream (nse2d_les_attach)
+    // mode 2: the channel calls the hook before each of the three stages of a step; only the
+    // first call of a step evaluates, and the channel's buffer keeps that term for the others
+    if (L->attach_mode == 2 && (L->hook_calls++ % 3) != 0) return;
     features(L, w, psi, L->Xb, L->norm);
     nse2d_cnn_infer(L->cnn, L->Xb, L->out, 1);
     const Grid& g = L->c;
@@ -508,6 +513,9 @@ int nse2d_les_closure(nse2d_les_t L, const double* d_omega_c, const double* d_ps
     return 0;
 }
 int nse2d_les_attach(nse2d_les_t L, nse2d_channel_t coarse_ch, int on) {
+    if (on < 0 || on > 2) { set_last_error("nse2d_les_attach: mode must be 0, 1 or 2"); return -1; }
+    L->impl->attach_mode = on;
+    L->impl->hook_calls = 0;
     return nse2d_channel_set_closure(coarse_ch, on ? closure_hook : nullptr, L->impl);
 }
 

@@ -4,10 +4,10 @@
   2. training pairs from its states: the filtered state and the closure target
      (nse2d_les_pair: box filter in mapped y, sharp spectral cut in x, FP64);
   3. Adam training of the CNN on the device (nse2d_les_train, bf16 MFMA kernels);
-  4. coarse runs from the filtered DNS state, with the closure attached (nse2d_les_attach) and
-     without it, whose mean profile U(y) and Re_tau are compared with those of the DNS over the
-     same time window, and whose wall-clock time per unit of simulated time is compared with the
-     DNS's.
+  4. coarse runs from the filtered DNS state, with the closure attached (nse2d_les_attach: in
+     every SSPRK stage, or once per step) and without it, whose mean profile U(y) and Re_tau are
+     compared with those of the DNS over the same time window, and whose wall-clock time per unit
+     of simulated time is compared with the DNS's.
 
 No reference counterpart (the reference runs DNS only). Nothing here runs on the CPU beyond the
 bookkeeping of a few profiles; there is no fallback.
@@ -182,8 +182,10 @@ class Les:
             raise RuntimeError(nh.last_error())
         return a.value, b.value
 
-    def attach(self, ch, on=True):
-        nh.lib().nse2d_les_attach(self.h, ch.ch, 1 if on else 0)
+    def attach(self, ch, mode=1):
+        """mode 1: closure of every stage state; 2: once per step; 0: detach."""
+        if nh.lib().nse2d_les_attach(self.h, ch.ch, int(mode)):
+            raise RuntimeError(nh.last_error())
 
     def destroy(self):
         if self.h:
@@ -274,13 +276,15 @@ def run(fine_cfg, coarse_cfg, *, spinup_time, window_time, n_pairs, holdout, tra
             f"{losses[:10].mean():.4f} -> {losses[-10:].mean():.4f}; hold-out normalised MSE {lc:.4f} "
             f"(zero closure {lz:.4f})")
         # 4. coarse runs from the first filtered state, compared at the sample times
-        les.attach(coarse_cl, True)
         U_dns = np.mean([s[2] for s in samples[1:]], axis=0)
         re_fine = float(np.mean([s[3] for s in samples[1:]]))
         re_filt = float(np.mean([s[4] for s in samples[1:]]))
         out["re_tau_dns_fine"], out["re_tau_dns_filtered"] = re_fine, re_filt
         res = {}
-        for name, ch in (("closure", coarse_cl), ("no_closure", coarse_nc)):
+        for name, ch, mode in (("closure", coarse_cl, 1), ("closure_per_step", coarse_cl, 2),
+                               ("no_closure", coarse_nc, 0)):
+            if mode:
+                les.attach(ch, mode)
             ch.set_omega(samples[0][1], t0)
             t = t0
             errs, U_acc, re_acc = [], [], []
@@ -317,6 +321,7 @@ def run(fine_cfg, coarse_cfg, *, spinup_time, window_time, n_pairs, holdout, tra
                 f"{r['wall_per_time']:.3f} s per unit time")
         out["runs"] = res
         out["speedup_vs_dns"] = out["dns_wall_per_time"] / res["closure"]["wall_per_time"]
+        out["speedup_vs_dns_per_step_closure"] = out["dns_wall_per_time"] / res["closure_per_step"]["wall_per_time"]
         return out
     finally:
         les.destroy()

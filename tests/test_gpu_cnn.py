@@ -102,7 +102,7 @@ def test_training_reduces_loss_on_a_fixed_batch():
     nh.lib().nse2d_cnn_destroy(c)
 
 
-@pytest.mark.parametrize("H,W,B", [(9, 64, 2), (7, 128, 1), (33, 64, 3)])
+@pytest.mark.parametrize("H,W,B", [(9, 64, 2), (7, 128, 1), (33, 64, 3), (257, 512, 2)])
 def test_staged_convolution_matches_direct(H, W, B):
     """The LDS-staged convolution accumulates the same products in the same order as the kernel
     that reads its operands from global memory (taps beyond the walls add exact zeros), so the
@@ -115,7 +115,7 @@ def test_staged_convolution_matches_direct(H, W, B):
     gout = rng.normal(size=(B, H, W)).astype(np.float32)
     d_in, d_out, d_g = nh.DeviceArray(inp), nh.DeviceArray(shape=(B, H, W)), nh.DeviceArray(gout)
     res = {}
-    for direct in (1, 0):
+    for direct in (1, 0, 2):  # global operands, persistent LDS-staged (default), per-tile LDS-staged
         assert nh.lib().nse2d_cnn_set_direct(c, direct) == 0
         assert nh.lib().nse2d_cnn_forward(c, d_in.ptr, d_out.ptr, B) == 0
         out = d_out.download()
@@ -125,6 +125,8 @@ def test_staged_convolution_matches_direct(H, W, B):
         assert nh.lib().nse2d_cnn_infer(c, d_in.ptr, d_out.ptr, B) == 0
         res[direct] = (out, g, d_out.download())
     nh.lib().nse2d_cnn_destroy(c)
-    assert np.array_equal(res[0][0], res[1][0])
-    assert np.array_equal(res[0][1], res[1][1])
-    assert np.array_equal(res[0][2], res[0][0]) and np.array_equal(res[1][2], res[1][0])
+    for m in (0, 2):
+        assert np.array_equal(res[m][0], res[1][0]), m
+        assert np.array_equal(res[m][1], res[1][1]), m
+    for m in (0, 1, 2):
+        assert np.array_equal(res[m][2], res[m][0]), m

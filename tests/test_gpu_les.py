@@ -246,6 +246,8 @@ def test_training_on_filtered_dns_beats_the_zero_closure():
     # a posteriori: started from the same filtered state, the coarse run with the closure follows
     # the filtered DNS more closely than the run without it (measured: 0.2 % against 3.9 % at
     # the end of the window), and its Re_tau is closer to the filtered DNS's
-    cl, nc = r["runs"]["closure"], r["runs"]["no_closure"]
+    cl, nc, cs = r["runs"]["closure"], r["runs"]["no_closure"], r["runs"]["closure_per_step"]
     assert cl["traj_err_end"] < 0.5 * nc["traj_err_end"]
     assert cl["re_tau_rel_err"] < nc["re_tau_rel_err"]
+    # the closure evaluated once per step (held over the three stages) keeps most of the gain
+    assert cs["finite"] and cs["traj_err_end"] < 0.5 * nc["traj_err_end"]
