@@ -7,6 +7,7 @@
 #include "nse2d_poisson.hpp"
 #include "../../include/nse2d_hip.h"
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <string>
@@ -308,6 +309,17 @@ __global__ void __launch_bounds__(64 * SYW) stage_kernel_wave_group(const StageA
     stage_wave_body<STAGE, RY, SYW, PF>(args[m], b - tile_begin[m]);
 }
 
+// The two-column stage of several channels in one launch (tiles as stage_kernel_wave_group, of
+// 128-column strips).
+template <int STAGE, int RY, int SYW, int PF>
+__global__ void __launch_bounds__(64 * SYW) stage_kernel_wave2_group(const StageArgs* __restrict__ args,
+                                                                     const int* __restrict__ tile_begin, int n) {
+    const int b = xcd_tile(blockIdx.x, gridDim.x);
+    int m = 0;
+    while (m + 1 < n && tile_begin[m + 1] <= b) ++m;
+    stage_wave2_body<STAGE, RY, SYW, PF>(args[m], b - tile_begin[m]);
+}
+
 // Forcing profile f(y_j) (ops.cu:526-531), evaluated once per row.
 __global__ void forcing_rows_kernel(double* f, const double* y, int Ny, double F0, int nforce, double h) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -369,6 +381,57 @@ __global__ void __launch_bounds__(kCflThreads) cfl_rows_kernel(const double* __r
     __shared__ double sm[3 * 16];
     row_max_uv_body(psi, Nx, Ny, inv_dx, inv_deta, a_node, xcd_tile(blockIdx.x, gridDim.x), partial, sm);
 }
+
+// The same maxima for Nx a multiple of 512: workgroup tiles of 512 columns x kCflRows interior rows
+// (rows 1 + kCflRows band ..), one 128-column strip per wavefront, lane l holding columns x0 + 2l
+// and x0 + 2l + 1 (16-byte loads); each wavefront walks down its rows with a window of three rows
+// in registers and the row after next already in flight. x-neighbours come from DPP shifts and the
+// strip's two halo columns. Workgroup b writes its maxima to partial[a * gridDim.x + b].
+constexpr int kCflRows = 16;
+__global__ void __launch_bounds__(kCflThreads) cfl_rows2_kernel(const double* __restrict__ psi, int Nx, int Ny,
+                                                                double inv_dx, double inv_deta,
+                                                                const double* __restrict__ a_node,
+                                                                double* __restrict__ partial) {
+    __shared__ double sm[3 * 16];
+    const int nsx = Nx / 512;
+    const int b = xcd_tile(blockIdx.x, gridDim.x);
+    const int sx = b % nsx, band = b / nsx;
+    const int j0 = 1 + band * kCflRows, j1 = min(j0 + kCflRows, Ny - 1);  // rows [j0, j1)
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int x0 = sx * 512 + wave * 128, ix = x0 + 2 * lane;
+    const bool edge = (lane == 0) || (lane == 63);
+    const int hx = (lane == 0) ? (x0 == 0 ? Nx - 1 : x0 - 1) : (x0 + 128 == Nx ? 0 : x0 + 128);
+    const double hdy = 0.5 * inv_deta, hdx = 0.5 * inv_dx;
+    auto row = [&](int j) { return *reinterpret_cast<const double2*>(psi + idx2(j, ix, Nx)); };
+    auto halo = [&](int j) { return edge ? psi[idx2(j, hx, Nx)] : 0.0; };
+    double2 pm = row(j0 - 1), pc = row(j0), pp = row(j0 + 1);
+    double hc = halo(j0), hp = (j0 + 1 < j1) ? halo(j0 + 1) : 0.0;
+    double m0 = 0.0, m1 = 0.0, m2 = 0.0;
+    for (int j = j0; j < j1; ++j) {
+        // the row after next, in flight while this row is reduced
+        double2 pn = pp;
+        double hn = 0.0;
+        if (j + 2 <= j1) { pn = row(j + 2); hn = (j + 2 < j1) ? halo(j + 2) : 0.0; }
+        const double a = a_node[j];
+        // u = a d psi / d eta (rows j -+ 1), v = -d psi / dx (columns x -+ 1), as diag.cu:740-760
+        const double l = wave_shr1(pc.y, hc), r = wave_shl1(pc.x, hc);  // columns x0+2l-1, x0+2l+2
+        const double u0 = fabs(a * ((pp.x - pm.x) * hdy)), u1 = fabs(a * ((pp.y - pm.y) * hdy));
+        const double v0 = fabs(-((pc.y - l) * hdx)), v1 = fabs(-((r - pc.x) * hdx));
+        m0 = fmax(m0, fmax(u0, u1));
+        m1 = fmax(m1, fmax(v0, v1));
+        m2 = fmax(m2, fmax(v0 * a, v1 * a));
+        pm = pc;
+        pc = pp;
+        pp = pn;
+        hc = hp;
+        hp = hn;
+    }
+    double m[3] = {m0, m1, m2};
+    block_max<3>(m, sm);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; ++k) partial[(size_t)k * gridDim.x + b] = m[k];
+}
+
 __global__ void __launch_bounds__(kCflThreads) cfl_final_kernel(const double* __restrict__ partial, int Ny,
                                                                 double* __restrict__ scal, DtRule r) {
     __shared__ double sm[3 * 16];
@@ -620,8 +683,13 @@ struct EnsMember {
     int cls = 0;
 };
 
-int stage_code_ry(int c) { return c / 100; }
+int stage_code_ry(int c) { return (c % 10000) / 100; }
 int stage_code_sy(int c) { return (c / 10) % 10; }
+bool stage_code_wide(int c) { return c >= 10000; }
+std::string stage_code_name(int c) {
+    return (stage_code_wide(c) ? "w" : "") + std::to_string(stage_code_ry(c)) + "x" + std::to_string(stage_code_sy(c)) +
+           "x" + std::to_string(c % 10);
+}
 
 }  // namespace
 
@@ -639,6 +707,7 @@ struct EnsembleImpl {
     double* d_partial = nullptr;
     int total_rows = 0;
     int stage_tile = kStageDefault;
+    bool all_wide = false;  // every member has Nx % 128 == 0 (two-column stage kernels allowed)
     bool use_graph = true;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -656,9 +725,10 @@ void ens_drop_graph(EnsembleImpl* e) {
 void ens_set_tiles(EnsembleImpl* e) {
     const int n = (int)e->mem.size();
     const int rows = stage_code_ry(e->stage_tile) * stage_code_sy(e->stage_tile);
+    const int strip = stage_code_wide(e->stage_tile) ? 128 : 64;
     std::vector<int> tb(n + 1, 0);
     for (int m = 0; m < n; ++m)
-        tb[m + 1] = tb[m] + (e->mem[m].d.Nx / 64) * (int)ceil_div(e->mem[m].d.Ny, rows);
+        tb[m + 1] = tb[m] + (e->mem[m].d.Nx / strip) * (int)ceil_div(e->mem[m].d.Ny, rows);
     e->total_tiles = tb[n];
     NSE2D_HIP_CHECK(hipMemcpyAsync(e->d_tile_begin, tb.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, e->s));
     NSE2D_HIP_CHECK(hipStreamSynchronize(e->s));
@@ -676,14 +746,26 @@ void ens_launch_stage(EnsembleImpl* e, int stage) {
         else                                                                                               \
             stage_kernel_wave_group<3, RYV, SYV, PFV><<<e->total_tiles, dim3(64, SYV), 0, e->s>>>(args, e->d_tile_begin, n); \
         break;
+#define NSE2D_ENS_WAVE2(RYV, SYV, PFV)                                                                     \
+    case 10000 + RYV * 100 + SYV * 10 + PFV:                                                               \
+        if (stage == 1)                                                                                    \
+            stage_kernel_wave2_group<1, RYV, SYV, PFV><<<e->total_tiles, dim3(64, SYV), 0, e->s>>>(args, e->d_tile_begin, n); \
+        else if (stage == 2)                                                                               \
+            stage_kernel_wave2_group<2, RYV, SYV, PFV><<<e->total_tiles, dim3(64, SYV), 0, e->s>>>(args, e->d_tile_begin, n); \
+        else                                                                                               \
+            stage_kernel_wave2_group<3, RYV, SYV, PFV><<<e->total_tiles, dim3(64, SYV), 0, e->s>>>(args, e->d_tile_begin, n); \
+        break;
     switch (e->stage_tile) {
         NSE2D_ENS_WAVE(8, 4, 1) NSE2D_ENS_WAVE(8, 4, 2) NSE2D_ENS_WAVE(8, 4, 3)
         NSE2D_ENS_WAVE(16, 4, 1) NSE2D_ENS_WAVE(16, 4, 2) NSE2D_ENS_WAVE(16, 4, 3)
         NSE2D_ENS_WAVE(16, 2, 1) NSE2D_ENS_WAVE(16, 2, 2) NSE2D_ENS_WAVE(16, 2, 3)
         NSE2D_ENS_WAVE(32, 1, 1) NSE2D_ENS_WAVE(32, 1, 2) NSE2D_ENS_WAVE(32, 1, 3)
+        NSE2D_ENS_WAVE2(8, 4, 1) NSE2D_ENS_WAVE2(8, 4, 2) NSE2D_ENS_WAVE2(4, 4, 1)
+        NSE2D_ENS_WAVE2(16, 2, 1) NSE2D_ENS_WAVE2(8, 2, 1)
         default: std::abort();
     }
 #undef NSE2D_ENS_WAVE
+#undef NSE2D_ENS_WAVE2
     NSE2D_LAUNCH_CHECK();
 }
 
@@ -865,9 +947,11 @@ EnsembleImpl* ens_create(const nse2d_channel_desc* descs, int n, hipStream_t s) 
     NSE2D_HIP_CHECK(hipMemcpyAsync(e->d_rules, rules.data(), sizeof(DtRule) * n, hipMemcpyHostToDevice, e->s));
     NSE2D_HIP_CHECK(hipMemcpyAsync(e->d_args, args.data(), sizeof(StageArgs) * 3 * n, hipMemcpyHostToDevice, e->s));
     NSE2D_HIP_CHECK(hipMemcpyAsync(e->d_jobs, jobs.data(), sizeof(MaxUVJob) * n, hipMemcpyHostToDevice, e->s));
+    e->all_wide = true;
+    for (const EnsMember& M : e->mem) e->all_wide = e->all_wide && (M.d.Nx % 128 == 0);
     e->stage_tile = stage_tile_code();
-    // the generic and the two-column kernels have no group form
-    if (e->stage_tile == 0 || e->stage_tile >= 10000) e->stage_tile = kStageDefault;
+    // the generic kernel has no group form; the two-column kernels need every Nx % 128 == 0
+    if (e->stage_tile == 0 || (stage_code_wide(e->stage_tile) && !e->all_wide)) e->stage_tile = kStageDefault;
     ens_set_tiles(e);
     return e;
 }
@@ -921,10 +1005,12 @@ std::string ens_autotune(EnsembleImpl* e, int nsteps) {
     NSE2D_HIP_CHECK(hipStreamSynchronize(e->s));
     std::string rep;
     {
-        const int codes[] = {841, 842, 843, 1641, 1642, 1643, 1621, 1622, 1623, 3211, 3212, 3213};
+        const int codes[] = {841,   842,   843,   1641,  1642,  1643,  1621, 1622, 1623, 3211, 3212, 3213,
+                             10821, 10841, 10842, 10441, 11621};
         int best = e->stage_tile;
         double tbest = 1e300;
         for (int code : codes) {
+            if (stage_code_wide(code) && !e->all_wide) continue;
             e->stage_tile = code;
             ens_set_tiles(e);
             const double t = ens_time_steps(e, nsteps);
@@ -932,8 +1018,7 @@ std::string ens_autotune(EnsembleImpl* e, int nsteps) {
         }
         e->stage_tile = best;
         ens_set_tiles(e);
-        rep += "stage " + std::to_string(best / 100) + "x" + std::to_string((best / 10) % 10) + "x" +
-               std::to_string(best % 10);
+        rep += "stage " + stage_code_name(best);
     }
     for (EnsClass& c : e->cls) {
         int best_r = poisson_fft_radix(c.P), best_k = poisson_thomas_kc(c.P);
@@ -1019,7 +1104,10 @@ nse2d_channel_t nse2d_channel_create(const nse2d_channel_desc* desc, hipStream_t
         NSE2D_HIP_CHECK(hipMemsetAsync(*f, 0, N * sizeof(double), s));
     }
     NSE2D_HIP_CHECK(hipMalloc(&c->f_row, d.Ny * sizeof(double)));
-    NSE2D_HIP_CHECK(hipMalloc(&c->cfl_partial, 3 * (size_t)d.Ny * sizeof(double)));
+    {   // one partial per row (cfl_rows_kernel) or per 512 x kCflRows tile (cfl_rows2_kernel)
+        const size_t tiles = (d.Nx >= 512 ? (size_t)(d.Nx / 512) : 1) * ceil_div(d.Ny > 2 ? d.Ny - 2 : 1, kCflRows);
+        NSE2D_HIP_CHECK(hipMalloc(&c->cfl_partial, 3 * std::max((size_t)d.Ny, tiles) * sizeof(double)));
+    }
     NSE2D_HIP_CHECK(hipMalloc(&c->scalars, 16 * sizeof(double)));
     NSE2D_HIP_CHECK(hipMemsetAsync(c->scalars, 0, 16 * sizeof(double), s));
     forcing_rows_kernel<<<ceil_div(d.Ny, 256), 256, 0, s>>>(c->f_row, d.d_y_node, d.Ny, d.F0, d.nforce, d.h);
@@ -1083,10 +1171,17 @@ void nse2d_channel_step_auto(nse2d_channel_t ch, int n) {
                 advance_time_kernel<<<1, 64, 0, c->s>>>(c->t_ptr(), c->dt_ptr());
             } else {
                 // t advances before the stages: nothing in the step reads it (no manufactured forcing)
-                cfl_rows_kernel<<<d.Ny, kCflThreads, 0, c->s>>>(c->psi, d.Nx, d.Ny, c->inv_dx, c->inv_deta, d.d_a_node,
-                                                                c->cfl_partial);
+                int nparts = d.Ny;
+                if (d.Nx % 512 == 0) {
+                    nparts = (d.Nx / 512) * (int)ceil_div(d.Ny - 2, kCflRows);
+                    cfl_rows2_kernel<<<nparts, kCflThreads, 0, c->s>>>(c->psi, d.Nx, d.Ny, c->inv_dx, c->inv_deta,
+                                                                       d.d_a_node, c->cfl_partial);
+                } else {
+                    cfl_rows_kernel<<<d.Ny, kCflThreads, 0, c->s>>>(c->psi, d.Nx, d.Ny, c->inv_dx, c->inv_deta,
+                                                                    d.d_a_node, c->cfl_partial);
+                }
                 NSE2D_LAUNCH_CHECK();
-                cfl_final_kernel<<<1, kCflThreads, 0, c->s>>>(c->cfl_partial, d.Ny, c->scalars, rule);
+                cfl_final_kernel<<<1, kCflThreads, 0, c->s>>>(c->cfl_partial, nparts, c->scalars, rule);
             }
             NSE2D_LAUNCH_CHECK();
         }

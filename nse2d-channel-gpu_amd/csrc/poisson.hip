@@ -501,6 +501,7 @@ void choose_thomas_layout(PoissonImpl* p, int want) {
     int pref = 8;
     if (const char* e = std::getenv("NSE2D_THOMAS_KC")) pref = std::atoi(e);
     if (want > 0) pref = want;
+
     for (int kc : {pref, 8, 4, 2}) {
         if (kc != 8 && kc != 4 && kc != 2) continue;
         const int nch = kThomasThreads / kc;
