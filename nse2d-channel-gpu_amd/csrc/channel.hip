@@ -480,6 +480,9 @@ struct ChannelImpl {
     // optional timing of kernel classes with events on the launch stream
     int stage_tile = 10821;  // stage kernel shape (stage_tile_code; 0: generic kernel)
     int prof_mask = 0;
+    int prof_every = 1;         // time only every prof_every-th step (events cost time themselves)
+    long long prof_steps = 0;   // steps since the last nse2d_channel_profile
+    bool prof_now = true;       // this step is timed
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
     std::vector<std::pair<int, size_t>> ev_marks;  // (class, index of the start event)
@@ -512,7 +515,7 @@ struct ProfScope {
         }
         return c->ev_pool[c->ev_used++];
     }
-    ProfScope(ChannelImpl* ch, int k) : c(ch), cls(k), on((ch->prof_mask >> k) & 1) {
+    ProfScope(ChannelImpl* ch, int k) : c(ch), cls(k), on(ch->prof_now && ((ch->prof_mask >> k) & 1)) {
         if (!on) return;
         c->ev_marks.push_back({cls, c->ev_used});
         NSE2D_HIP_CHECK(hipEventRecord(next_event(), c->s));
@@ -632,6 +635,8 @@ void close_stage(ChannelImpl* c, double* f, int stage, bool bookkeeping) {
 void closure_term(ChannelImpl* c, const double* ws) {
     if (c->closure_fn) c->closure_fn(c->closure_ctx, ws, c->psi, c->closure_buf, c->s);
 }
+
+void next_step_timed(ChannelImpl* c) { c->prof_now = (c->prof_steps++ % c->prof_every) == 0; }
 
 void do_step(ChannelImpl* c, double t, double dt, const double* dt_dev, int flags) {
     const bool book = (flags & NSE2D_STEP_WALL_BOOKKEEPING) != 0;
@@ -1148,6 +1153,7 @@ void nse2d_channel_init(nse2d_channel_t ch) {
 }
 
 void nse2d_channel_step(nse2d_channel_t ch, double t, double dt, int flags) {
+    next_step_timed(&ch->impl);
     do_step(&ch->impl, t, dt, nullptr, flags);
 }
 
@@ -1161,6 +1167,7 @@ void nse2d_channel_step_auto(nse2d_channel_t ch, int n) {
     const double inv_dymin2 = 1.0 / (d.dy_min * d.dy_min);
     const DtRule rule{c->dx, c->deta, d.cfl, d.cvisc, d.nu, c->inv_dx2, inv_dymin2, d.dt_max};
     for (int k = 0; k < n; ++k) {
+        next_step_timed(c);
         double* m = c->max_ptr();
         {
             ProfScope ps(c, NSE2D_PROF_CFL);
@@ -1192,11 +1199,16 @@ void nse2d_channel_step_auto(nse2d_channel_t ch, int n) {
 void nse2d_channel_profile(nse2d_channel_t ch, int class_mask) {
     ChannelImpl* c = &ch->impl;
     c->prof_mask = class_mask;
+    c->prof_steps = 0;
     for (int k = 0; k < NSE2D_PROF_NCLASS; ++k) { c->prof_ms[k] = 0; c->prof_n[k] = 0; }
     c->ev_used = 0;
     c->ev_marks.clear();
 }
 
+void nse2d_channel_profile_every(nse2d_channel_t ch, int every) {
+    ch->impl.prof_every = every > 1 ? every : 1;
+    ch->impl.prof_steps = 0;
+}
 void nse2d_channel_profile_read(nse2d_channel_t ch, double* ms, long long* count) {
     ChannelImpl* c = &ch->impl;
     NSE2D_HIP_CHECK(hipStreamSynchronize(c->s));

@@ -76,6 +76,7 @@ SIGNATURES = {
     "nse2d_channel_dt_ptr": (_vp, [_vp]),
     "nse2d_channel_wall_bookkeeping": (None, [_vp, _p, _d]),
     "nse2d_channel_profile": (None, [_vp, _i]),
+    "nse2d_channel_profile_every": (None, [_vp, _i]),
     "nse2d_channel_profile_read": (None, [_vp, _p, ctypes.POINTER(ctypes.c_longlong)]),
     # spectra
     "nse2d_spectra1d_create": (_vp, [_i, _i, _d, _stream]),

@@ -312,3 +312,49 @@ def test_device_dt_matches_host_rule():
     assert abs(t_dev - t) <= 1e-14
     assert rel_err(field(ch, nh.FIELD_OMEGA, g), w_o) <= 1e-10
     nh.lib().nse2d_channel_destroy(ch)
+
+
+def test_kernel_class_timing_and_sampling():
+    """nse2d_channel_profile brackets every launch of the chosen kernel classes with events;
+    nse2d_channel_profile_every(k) times only steps 0, k, 2k, ... after the reset. Timing leaves the
+    fields bit for bit as an untimed run makes them."""
+    Nx, Ny = 256, 97
+    g = Grid(Nx, Ny, 2.0)
+    w0 = channel_ic(Nx, Ny)
+    chans = []
+    for _ in range(2):
+        ch = make_channel(g, 2e-4, 0.05, 12, 1e-3, 2.0)
+        nh.upload_ptr(nh.lib().nse2d_channel_field(ch, nh.FIELD_OMEGA), w0)
+        nh.lib().nse2d_channel_init(ch)
+        chans.append(ch)
+    timed, plain = chans
+    ncls = len(nh.PROF_CLASSES)
+    L = nh.lib()
+
+    def read(ch):
+        ms, cnt = (ctypes.c_double * ncls)(), (ctypes.c_longlong * ncls)()
+        L.nse2d_channel_profile_read(ch, ms, cnt)
+        return np.array(ms[:]), np.array(cnt[:], dtype=np.int64)
+
+    L.nse2d_channel_profile(timed, (1 << ncls) - 1)
+    L.nse2d_channel_step_auto(timed, 4)
+    ms, n = read(timed)
+    per_step = n // 4
+    print(f"[profile] launches per step {dict(zip(nh.PROF_CLASSES, per_step.tolist()))}", flush=True)
+    assert np.array_equal(per_step * 4, n)
+    for cls in ("stage", "fft_forward", "thomas", "fft_inverse"):
+        k = nh.PROF_CLASSES.index(cls)
+        assert per_step[k] == 3 and ms[k] > 0.0
+    assert per_step[nh.PROF_CLASSES.index("cfl")] == 1
+    L.nse2d_channel_profile(timed, (1 << ncls) - 1)
+    L.nse2d_channel_profile_every(timed, 3)
+    L.nse2d_channel_step_auto(timed, 7)  # steps 0, 3 and 6 of this segment are timed
+    ms, n = read(timed)
+    assert np.array_equal(n, per_step * 3), (n, per_step)
+    L.nse2d_channel_profile(timed, 0)
+    L.nse2d_channel_profile_every(timed, 1)
+    L.nse2d_channel_step_auto(plain, 11)
+    sync()
+    assert np.array_equal(field(timed, nh.FIELD_OMEGA, g), field(plain, nh.FIELD_OMEGA, g))
+    for ch in chans:
+        L.nse2d_channel_destroy(ch)
