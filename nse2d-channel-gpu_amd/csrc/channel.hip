@@ -5,6 +5,7 @@
 #include "nse2d_common.hpp"
 #include "nse2d_ops.hpp"
 #include "nse2d_poisson.hpp"
+#include "fft_rows.hpp"
 #include "../../include/nse2d_hip.h"
 
 #include <algorithm>
@@ -194,6 +195,73 @@ __device__ __forceinline__ void stage_wave_body(const StageArgs& a, int tile) {
 // per lane (1 KB per wavefront instruction). The x-neighbours outside the pair come from the DPP
 // shifts of the other element (lane l-1's right value, lane l+1's left value), the strip's halo
 // columns from lanes 0 and 63.
+// One row of a 128-column strip as loaded: the lane's pair of psi and omega_s, and (lanes 0 and
+// 63) the strip's halo column.
+struct PairRow { double2 p, w; double ph, wh; };
+// (x0+2l-1, x0+2l, x0+2l+1, x0+2l+2) of a loaded row
+struct Row4 { double v[4]; };
+
+__device__ __forceinline__ void load_pair_row(const StageArgs& a, int j, int ix, bool edge, int hx, PairRow& r) {
+    const double* __restrict__ pr = a.psi + (size_t)j * a.Nx;
+    const double* __restrict__ wr = a.ws + (size_t)j * a.Nx;
+    r.p = *reinterpret_cast<const double2*>(pr + ix);
+    r.w = *reinterpret_cast<const double2*>(wr + ix);
+    r.ph = 0.0;
+    r.wh = 0.0;
+    if (edge) { r.ph = pr[hx]; r.wh = wr[hx]; }
+}
+__device__ __forceinline__ void expand_pair_row(const PairRow& r, Row4& P4, Row4& W4) {
+    P4.v[0] = wave_shr1(r.p.y, r.ph); P4.v[1] = r.p.x; P4.v[2] = r.p.y; P4.v[3] = wave_shl1(r.p.x, r.ph);
+    W4.v[0] = wave_shr1(r.w.y, r.wh); W4.v[1] = r.w.x; W4.v[2] = r.w.y; W4.v[3] = wave_shl1(r.w.x, r.wh);
+}
+__device__ __forceinline__ Stencil9 stencil_at(const Row4& m, const Row4& c, const Row4& pl, int e) {
+    Stencil9 S;
+    S.xm = c.v[e]; S.c = c.v[e + 1]; S.xp = c.v[e + 2];
+    S.mm = m.v[e]; S.ym = m.v[e + 1]; S.pm = m.v[e + 2];
+    S.mp = pl.v[e]; S.yp = pl.v[e + 1]; S.pp = pl.v[e + 2];
+    return S;
+}
+
+// Row j's metric factors and forcing (zero metrics on the wall rows, which take no J or Laplacian).
+struct RowCoef { double an, ae, aw, fr; };
+__device__ __forceinline__ RowCoef row_coef(const StageArgs& a, int j) {
+    const bool interior = (j > 0 && j < a.Ny - 1);
+    return {interior ? a.a_node[j] : 0.0, interior ? a.a_edge[j] : 0.0, interior ? a.a_edge[j - 1] : 0.0,
+            a.f_row[j]};
+}
+
+// The two updated values (columns ix, ix+1) of row j from the expanded rows j-1, j, j+1 of psi
+// and omega_s and the lane's pair of omega^n (stages 2, 3); stores the RHS copy to rhs_out when
+// it is not null.
+template <int STAGE>
+__device__ __forceinline__ double2 stage_pair(const StageArgs& a, int j, int ix, const Row4& Pm, const Row4& Pc,
+                                              const Row4& Pp, const Row4& Wm, const Row4& Wc, const Row4& Wp,
+                                              double2 wn_c, double dt, double inv4dxde, const RowCoef& rk,
+                                              double* __restrict__ rhs_out) {
+    double o[2];
+    const bool interior = (j > 0 && j < a.Ny - 1);
+    const double an = rk.an, ae = rk.ae, aw = rk.aw, fr = rk.fr;
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+        double J = 0.0, lap = 0.0;
+        const Stencil9 P = stencil_at(Pm, Pc, Pp, e), W = stencil_at(Wm, Wc, Wp, e);
+        if (interior) {
+            double J1, J2, J3;
+            arakawa_terms(P, W, an, inv4dxde, J1, J2, J3);
+            J = -((J1 + J2 + J3) * (1.0 / 3.0));
+            lap = mapped_laplacian(W.xm, W.c, W.xp, W.ym, W.yp, an, ae, aw, a.inv_dx2, a.inv_deta2);
+        }
+        const double wc = W.c;
+        const double rhs = -J + a.nu * lap + fr - a.lin_drag * wc;
+        if (rhs_out) rhs_out[idx2(j, ix + e, a.Nx)] = rhs;
+        const double wn = e ? wn_c.y : wn_c.x;
+        if (STAGE == 1) o[e] = wc + dt * rhs;
+        else if (STAGE == 2) o[e] = 0.75 * wn + 0.25 * (wc + dt * rhs);
+        else o[e] = (1.0 / 3.0) * wn + (2.0 / 3.0) * (wc + dt * rhs);
+    }
+    return double2{o[0], o[1]};
+}
+
 template <int STAGE, int RY, int SYW, int PF>
 __device__ __forceinline__ void stage_wave2_body(const StageArgs& a, int tile) {
     const int Nx = a.Nx, Ny = a.Ny;
@@ -207,84 +275,98 @@ __device__ __forceinline__ void stage_wave2_body(const StageArgs& a, int tile) {
     const int hx = (lane == 0) ? (x0 == 0 ? Nx - 1 : x0 - 1) : (x0 + 128 == Nx ? 0 : x0 + 128);
     const double inv4dxde = 0.25 * a.inv_dx * a.inv_deta;
     const double dt = a.dt_ptr ? *a.dt_ptr : a.dt_val;
-    struct Raw { double2 p, w; double ph, wh; };
-    auto load = [&](int j, Raw& r) {
-        const double* __restrict__ pr = a.psi + (size_t)j * Nx;
-        const double* __restrict__ wr = a.ws + (size_t)j * Nx;
-        r.p = *reinterpret_cast<const double2*>(pr + ix);
-        r.w = *reinterpret_cast<const double2*>(wr + ix);
-        r.ph = 0.0;
-        r.wh = 0.0;
-        if (edge) { r.ph = pr[hx]; r.wh = wr[hx]; }
-    };
-    // (x0+2l-1, x0+2l, x0+2l+1, x0+2l+2) of a loaded row
-    struct Row4 { double v[4]; };
-    auto expand = [&](const Raw& r, Row4& P4, Row4& W4) {
-        P4.v[0] = wave_shr1(r.p.y, r.ph); P4.v[1] = r.p.x; P4.v[2] = r.p.y; P4.v[3] = wave_shl1(r.p.x, r.ph);
-        W4.v[0] = wave_shr1(r.w.y, r.wh); W4.v[1] = r.w.x; W4.v[2] = r.w.y; W4.v[3] = wave_shl1(r.w.x, r.wh);
-    };
-    auto stencil = [&](const Row4& m, const Row4& c, const Row4& pl, int e) {
-        Stencil9 S;
-        S.xm = c.v[e]; S.c = c.v[e + 1]; S.xp = c.v[e + 2];
-        S.mm = m.v[e]; S.ym = m.v[e + 1]; S.pm = m.v[e + 2];
-        S.mp = pl.v[e]; S.yp = pl.v[e + 1]; S.pp = pl.v[e + 2];
-        return S;
-    };
     Row4 Pm{}, Pc{}, Pp{}, Wm{}, Wc{}, Wp{};
-    Raw ring[PF];
+    PairRow ring[PF];
     double2 wring[PF];
 #pragma unroll
     for (int f = 0; f < PF; ++f) {
-        if (f < RY && jlo + 1 + f < Ny) load(jlo + 1 + f, ring[f]);
+        if (f < RY && jlo + 1 + f < Ny) load_pair_row(a, jlo + 1 + f, ix, edge, hx, ring[f]);
         if (STAGE > 1 && f < RY && jlo + f < jhi)
             wring[f] = *reinterpret_cast<const double2*>(a.wn + idx2(jlo + f, ix, Nx));
     }
     if (jlo > 0) {
-        Raw r;
-        load(jlo - 1, r);
-        expand(r, Pm, Wm);
+        PairRow r;
+        load_pair_row(a, jlo - 1, ix, edge, hx, r);
+        expand_pair_row(r, Pm, Wm);
     }
     {
-        Raw r;
-        load(jlo, r);
-        expand(r, Pc, Wc);
+        PairRow r;
+        load_pair_row(a, jlo, ix, edge, hx, r);
+        expand_pair_row(r, Pc, Wc);
     }
 #pragma unroll
     for (int q = 0; q < RY; ++q) {
         const int j = jlo + q;
         if (j >= jhi) break;
-        const Raw cur = ring[q % PF];  // row j+1
+        const PairRow cur = ring[q % PF];  // row j+1
         const double2 wn_c = (STAGE > 1) ? wring[q % PF] : double2{0.0, 0.0};
-        if (q + PF < RY && j + 1 + PF < Ny) load(j + 1 + PF, ring[q % PF]);
+        if (q + PF < RY && j + 1 + PF < Ny) load_pair_row(a, j + 1 + PF, ix, edge, hx, ring[q % PF]);
         if (STAGE > 1 && q + PF < RY && j + PF < jhi)
             wring[q % PF] = *reinterpret_cast<const double2*>(a.wn + idx2(j + PF, ix, Nx));
-        if (j + 1 < Ny) expand(cur, Pp, Wp);
-        double o[2];
-        const bool interior = (j > 0 && j < Ny - 1);
-        const double an = interior ? a.a_node[j] : 0.0;
-        const double ae = interior ? a.a_edge[j] : 0.0, aw = interior ? a.a_edge[j - 1] : 0.0;
-        const double fr = a.f_row[j];
-#pragma unroll
-        for (int e = 0; e < 2; ++e) {
-            double J = 0.0, lap = 0.0;
-            const Stencil9 P = stencil(Pm, Pc, Pp, e), W = stencil(Wm, Wc, Wp, e);
-            if (interior) {
-                double J1, J2, J3;
-                arakawa_terms(P, W, an, inv4dxde, J1, J2, J3);
-                J = -((J1 + J2 + J3) * (1.0 / 3.0));
-                lap = mapped_laplacian(W.xm, W.c, W.xp, W.ym, W.yp, an, ae, aw, a.inv_dx2, a.inv_deta2);
-            }
-            const double wc = W.c;
-            const double rhs = -J + a.nu * lap + fr - a.lin_drag * wc;
-            if (a.rhs_out) a.rhs_out[idx2(j, ix + e, Nx)] = rhs;
-            const double wn = e ? wn_c.y : wn_c.x;
-            if (STAGE == 1) o[e] = wc + dt * rhs;
-            else if (STAGE == 2) o[e] = 0.75 * wn + 0.25 * (wc + dt * rhs);
-            else o[e] = (1.0 / 3.0) * wn + (2.0 / 3.0) * (wc + dt * rhs);
-        }
-        *reinterpret_cast<double2*>(a.out + idx2(j, ix, Nx)) = double2{o[0], o[1]};
+        if (j + 1 < Ny) expand_pair_row(cur, Pp, Wp);
+        *reinterpret_cast<double2*>(a.out + idx2(j, ix, Nx)) =
+            stage_pair<STAGE>(a, j, ix, Pm, Pc, Pp, Wm, Wc, Wp, wn_c, dt, inv4dxde, row_coef(a, j), a.rhs_out);
         Pm = Pc; Pc = Pp; Wm = Wc; Wc = Wp;
     }
+}
+
+// The stage and the forward row transform of the Poisson solve in one kernel, one workgroup per
+// grid row (rows contiguous per XCD, so that the rows j-1 and j+1 each workgroup reads are the
+// neighbouring workgroups' rows and meet in that XCD's L2). Thread tid holds the transform's
+// first-pass inputs z_{tid + q T} = (x_{2(tid + q T)}, x_{2(tid + q T) + 1}), q < R: in slot q the
+// wavefronts cover 128-column strips, so each slot is one row of the two-column stage (same
+// arithmetic, stage_pair). The updated row is stored, and on interior rows it goes from
+// registers through the row transform into the spectrum: the solve's forward phase then needs
+// no second read of omega_{s+1}. Production stages only (no manufactured forcing, closure term
+// or RHS copy).
+template <int STAGE, int M, int R>
+__global__ void __launch_bounds__(M / R) stage_fft_kernel(StageArgs a, fft::cplx* __restrict__ hat,
+                                                          const fft::cplx* __restrict__ twp,
+                                                          const fft::cplx* __restrict__ tw) {
+    constexpr int T = M / R;
+    __shared__ fft::cplx lds[fft::Plan<M, R>::LDS];
+    const int Nx = 2 * M, Ny = a.Ny;
+    const int j = xcd_tile(blockIdx.x, gridDim.x);
+    const int tid = threadIdx.x, lane = tid & 63;
+    const bool edge = (lane == 0) || (lane == 63);
+    const bool interior = j > 0 && j < Ny - 1;  // uniform over the workgroup
+    const double inv4dxde = 0.25 * a.inv_dx * a.inv_deta;
+    const double dt = a.dt_ptr ? *a.dt_ptr : a.dt_val;
+    // the row factors, loaded once (the compiler cannot tell that the stores below leave them alone)
+    const RowCoef rk = row_coef(a, j);
+    // slot q: the wavefront's 128-column strip x0 = 2 (q T + 64 w)
+    auto strip = [&](int q) { return 2 * (q * T + (tid & ~63)); };
+    auto halo = [&](int x0) { return (lane == 0) ? (x0 == 0 ? Nx - 1 : x0 - 1) : (x0 + 128 == Nx ? 0 : x0 + 128); };
+    struct Slot { PairRow m, c, p; double2 wn; };
+    auto load_slot = [&](int q, Slot& S) {
+        const int x0 = strip(q), ix = x0 + 2 * lane, hx = halo(x0);
+        load_pair_row(a, j, ix, edge, hx, S.c);
+        if (interior) {
+            load_pair_row(a, j - 1, ix, edge, hx, S.m);
+            load_pair_row(a, j + 1, ix, edge, hx, S.p);
+        }
+        S.wn = (STAGE > 1) ? *reinterpret_cast<const double2*>(a.wn + idx2(j, ix, Nx)) : double2{0.0, 0.0};
+    };
+    fft::cplx v[R];
+    Slot ring[2];
+    load_slot(0, ring[0]);
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        // the next slot's loads are in flight while this slot is computed
+        if (q + 1 < R) load_slot(q + 1, ring[(q + 1) & 1]);
+        const Slot& S = ring[q & 1];
+        const int ix = strip(q) + 2 * lane;
+        Row4 Pm{}, Pc{}, Pp{}, Wm{}, Wc{}, Wp{};
+        expand_pair_row(S.c, Pc, Wc);
+        if (interior) {
+            expand_pair_row(S.m, Pm, Wm);
+            expand_pair_row(S.p, Pp, Wp);
+        }
+        const double2 o = stage_pair<STAGE>(a, j, ix, Pm, Pc, Pp, Wm, Wc, Wp, S.wn, dt, inv4dxde, rk, nullptr);
+        *reinterpret_cast<double2*>(a.out + idx2(j, ix, Nx)) = o;
+        v[q] = {o.x, o.y};
+    }
+    if (interior) fft::forward_row_from<M, R>(v, hat + (size_t)j * M, lds, twp, tw);
 }
 
 template <int STAGE, int RY, int SYW, int PF>
@@ -479,6 +561,11 @@ struct ChannelImpl {
     double inv_dx = 0, inv_deta = 0, inv_dx2 = 0, inv_deta2 = 0, dx = 0, deta = 0;
     // optional timing of kernel classes with events on the launch stream
     int stage_tile = 10821;  // stage kernel shape (stage_tile_code; 0: generic kernel)
+    // NSE2D_STAGE_FFT=1: production stages with the two-column kernel run fused with the forward
+    // row transform (stage_fft_kernel) where the plan allows it. Off by default: at 4096x2049 the
+    // fused kernel moves the fewest bytes (340 MB per launch against 246 + 134 MB for the pair)
+    // but takes 84 us against 49 + 29 us, held to 3 wavefronts per SIMD by its 156 VGPRs.
+    bool stage_fft = false;
     int prof_mask = 0;
     int prof_every = 1;         // time only every prof_every-th step (events cost time themselves)
     long long prof_steps = 0;   // steps since the last nse2d_channel_profile
@@ -610,9 +697,9 @@ void launch_stage(ChannelImpl* c, const StageArgs& a) {
     NSE2D_LAUNCH_CHECK();
 }
 
-void close_stage(ChannelImpl* c, double* f, int stage, bool bookkeeping) {
+void close_stage(ChannelImpl* c, double* f, int stage, bool bookkeeping, bool forward_done) {
     const nse2d_channel_desc& d = c->d;
-    { ProfScope ps(c, NSE2D_PROF_FFT_FWD); poisson_forward_interior(c->P, f, c->s); }
+    if (!forward_done) { ProfScope ps(c, NSE2D_PROF_FFT_FWD); poisson_forward_interior(c->P, f, c->s); }
     { ProfScope ps(c, NSE2D_PROF_THOMAS); poisson_tridiag(c->P, d.psi_bot, d.psi_top, c->s); }
     if (!bookkeeping && poisson_fuses_closure(c->P)) {
         // the closure rides in the inverse transform's workgroups of rows 1 and Ny-2
@@ -638,18 +725,55 @@ void closure_term(ChannelImpl* c, const double* ws) {
 
 void next_step_timed(ChannelImpl* c) { c->prof_now = (c->prof_steps++ % c->prof_every) == 0; }
 
+bool stage_fft_plan(int M, int R) {
+    switch (M * 100 + R) {
+        case 256 * 100 + 4: case 512 * 100 + 8: case 1024 * 100 + 8: case 2048 * 100 + 8: case 4096 * 100 + 16:
+        case 8192 * 100 + 16: return true;
+        default: return false;
+    }
+}
+
+// The fused stage + forward row transform when it applies (production stage, two-column stage
+// shape, an instantiated row-FFT plan); false, having launched nothing, otherwise.
+template <int STAGE>
+bool launch_stage_fft(ChannelImpl* c, const StageArgs& a) {
+    if (!c->stage_fft || c->stage_tile < 10000 || a.mms || a.closure || a.rhs_out) return false;
+    int M = 0, R = 0;
+    const void *twp = nullptr, *tw = nullptr;
+    if (!poisson_row_fft(c->P, &M, &R, &twp, &tw) || !stage_fft_plan(M, R) || 2 * M != c->d.Nx) return false;
+    auto* hat = static_cast<fft::cplx*>(poisson_hat(c->P));
+    const auto* tp = static_cast<const fft::cplx*>(twp);
+    const auto* tt = static_cast<const fft::cplx*>(tw);
+    ProfScope ps(c, NSE2D_PROF_STAGE);
+    switch (M * 100 + R) {
+#define NSE2D_STAGE_FFT(MV, RV)                                                                         \
+    case MV * 100 + RV:                                                                                 \
+        stage_fft_kernel<STAGE, MV, RV><<<c->d.Ny, MV / RV, 0, c->s>>>(a, hat, tp, tt);                 \
+        break;
+        NSE2D_STAGE_FFT(256, 4) NSE2D_STAGE_FFT(512, 8) NSE2D_STAGE_FFT(1024, 8) NSE2D_STAGE_FFT(2048, 8)
+        NSE2D_STAGE_FFT(4096, 16) NSE2D_STAGE_FFT(8192, 16)
+#undef NSE2D_STAGE_FFT
+    }
+    NSE2D_LAUNCH_CHECK();
+    return true;
+}
+
+template <int STAGE>
+void stage_and_solve(ChannelImpl* c, const StageArgs& a, int stage, bool book) {
+    const bool fused = launch_stage_fft<STAGE>(c, a);
+    if (!fused) launch_stage<STAGE>(c, a);
+    close_stage(c, a.out, stage, book, fused);
+}
+
 void do_step(ChannelImpl* c, double t, double dt, const double* dt_dev, int flags) {
     const bool book = (flags & NSE2D_STEP_WALL_BOOKKEEPING) != 0;
     double* rhs3 = (flags & NSE2D_STEP_STORE_RHS) ? c->rhs : nullptr;
     closure_term(c, c->w);
-    launch_stage<1>(c, stage_args(c, c->w, nullptr, c->w1, nullptr, dt_dev, dt, t));
-    close_stage(c, c->w1, 0, book);
+    stage_and_solve<1>(c, stage_args(c, c->w, nullptr, c->w1, nullptr, dt_dev, dt, t), 0, book);
     closure_term(c, c->w1);
-    launch_stage<2>(c, stage_args(c, c->w1, c->w, c->w2, nullptr, dt_dev, dt, t + dt));
-    close_stage(c, c->w2, 1, book);
+    stage_and_solve<2>(c, stage_args(c, c->w1, c->w, c->w2, nullptr, dt_dev, dt, t + dt), 1, book);
     closure_term(c, c->w2);
-    launch_stage<3>(c, stage_args(c, c->w2, c->w, c->w, rhs3, dt_dev, dt, t + 0.5 * dt));
-    close_stage(c, c->w, 2, book);
+    stage_and_solve<3>(c, stage_args(c, c->w2, c->w, c->w, rhs3, dt_dev, dt, t + 0.5 * dt), 2, book);
 }
 
 }  // namespace
@@ -1096,6 +1220,7 @@ nse2d_channel_t nse2d_channel_create(const nse2d_channel_desc* desc, hipStream_t
     c->s = s;
     c->P = P;
     c->stage_tile = stage_tile_code();
+    if (const char* e = std::getenv("NSE2D_STAGE_FFT")) c->stage_fft = (e[0] == '1');
     c->dx = d.Lx / (double)d.Nx;
     c->inv_dx = 1.0 / c->dx;
     c->inv_dx2 = c->inv_dx * c->inv_dx;

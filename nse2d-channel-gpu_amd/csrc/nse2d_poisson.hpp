@@ -42,6 +42,10 @@ size_t poisson_hat_bytes(const PoissonImpl* p);
 // The plan's own spectrum buffer ([Ny][Nx/2] complex, slot 0 of a row holds (X_0, X_{Nx/2})):
 // written by poisson_forward, read by poisson_inverse.
 void* poisson_hat(PoissonImpl* p);
+// The register row FFT of plan p, for a kernel that transforms rows it produces itself (the
+// fused stage + forward transform): M = Nx/2, the radix R and the twiddle tables (passes >= 1,
+// and the real-split factors exp(-2 pi i k / Nx)). False for the small-row LDS plans.
+bool poisson_row_fft(const PoissonImpl* p, int* M, int* R, const void** twp, const void** tw);
 int poisson_group_supported(const PoissonImpl* p);
 void poisson_forward_group(PoissonImpl* p, const double* const* d_omegas, void* const* d_hats, int n,
                            hipStream_t s);

@@ -174,6 +174,18 @@ __global__ void row_coeff_kernel(double* __restrict__ a_sub, double* __restrict_
     c_sup[r] = a_node[j] * a_edge[j] * inv_deta2;
 }
 
+// 1/x for a pivot of the y-direction systems (|x| >= a + c > 0, far inside the normal range):
+// the hardware reciprocal refined by two Newton steps, within an ulp of the quotient. The IEEE
+// division adds scaling and fix-up steps for denormals and overflow, which a pivot never is.
+// Every kernel that forms a pivot uses this, so their recurrences agree bit for bit.
+__device__ __forceinline__ double pivot_recip(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    double e = __builtin_fma(-x, r, 1.0);
+    r = __builtin_fma(r, e, r);
+    e = __builtin_fma(-x, r, 1.0);
+    return __builtin_fma(r, e, r);
+}
+
 // Elimination recurrence of poisson.cu:38-68 (cp_r = c_r / (b_r - a_r cp_{r-1}), b_r =
 // -(a_r + c_r) - kx^2), one thread per kx (k = 0..M, k = M the Nyquist), storing the value that
 // enters each chunk: cpb[c][k] = cp_{c*RC - 1} for c >= 1. Same operations as the solve kernel,
@@ -191,7 +203,7 @@ __global__ void chunk_boundary_cp_kernel(double* __restrict__ cpb, const double*
             if (r >= Ni) break;
             const double a = a_sub[r], cc = c_sup[r];
             const double b = -(a + cc) - kx2[k];
-            const double gi = 1.0 / (b - a * cp_prev);
+            const double gi = pivot_recip(b - a * cp_prev);
             cp_prev = cc * gi;
         }
     }
@@ -204,7 +216,7 @@ __global__ void column_g_kernel(double* __restrict__ gcol, const double* __restr
     double cp_prev = 0.0;
     for (int r = 0; r < Ni; ++r) {
         const double a = a_sub[r], cc = c_sup[r];
-        const double gi = 1.0 / ((-(a + cc) - kk) - a * cp_prev);
+        const double gi = pivot_recip((-(a + cc) - kk) - a * cp_prev);
         gcol[r] = gi;
         cp_prev = cc * gi;
     }
@@ -234,126 +246,182 @@ struct ThomasShared {
     double ca[COEF ? (T / KC) * RC : 1], cc[COEF ? (T / KC) * RC : 1];
 };
 
-// The solve of column group `grp` (columns grp*KC .. grp*KC+KC-1) of one spectrum.
-template <int RC, int KC, int T>
+// The solve of column group `grp` (columns grp*KC .. grp*KC+KC-1) of one spectrum. SPLIT: the
+// group holds slot 0, whose imaginary part (Nyquist) has its own coefficient chain; in every
+// other group the imaginary parts share the real parts' chains (P, W, RP, Q), computed once.
+//
+// Issue order: the small coefficient loads first, then the lane's RC column values,
+// unconditionally (a row past the last interior row re-reads that row and is skipped by the
+// sweeps). Loads complete in issue order, so the coefficient staging waits only for the
+// coefficient loads and each elimination row only for its own column value: the forward sweep
+// runs while the column is still arriving. (Conditional loads, or loads issued after the
+// column, make the compiler wait for the whole column first.)
+template <int RC, int KC, int T, bool SPLIT>
 __device__ __forceinline__ void thomas_group(ThomasShared<RC, KC, T>& shm, int grp, cplx* __restrict__ hat, int M, int Ni,
                                     const double* __restrict__ a_sub, const double* __restrict__ c_sup,
                                     const double* __restrict__ kx2, const double* __restrict__ cpb,
                                     const double* __restrict__ gny, double bc_bot, double bc_top) {
     constexpr int NCH = T / KC;
+    constexpr int NR = NCH * RC;  // rows covered by the chunks (>= Ni)
+    constexpr bool COEF = ThomasShared<RC, KC, T>::COEF;
     auto& sm = shm.sm;
     auto& gq = shm.gq;
     const int t = threadIdx.x, kc = t % KC, c = t / KC;
     const int k = grp * KC + kc;
     const bool colok = k < M;
-    const bool split = (k == 0);
-    constexpr bool COEF = ThomasShared<RC, KC, T>::COEF;
+    const bool split = SPLIT && (k == 0);
     const int ldc = M + 1;
     const int r0 = c * RC;
     const int nr = colok ? max(0, min(RC, Ni - r0)) : 0;
-    // the column loads first: they stream while the coefficients are staged
-    cplx f[RC];
-    double g[RC];
+    const int kl = colok ? k : M - 1;
+    // coefficients: row coefficients (and the Nyquist g) of the rows this lane stages
+    constexpr int NS = (NR + T - 1) / T;
+    double sa[COEF ? NS : 1], sc[COEF ? NS : 1], sg[SPLIT ? NS : 1];
+    if (COEF || SPLIT) {
 #pragma unroll
-    for (int i = 0; i < RC; ++i)
-        if (i < nr) f[i] = hat[(size_t)(r0 + i + 1) * M + k];
-    const double kkx = colok ? kx2[k] : 0.0;
-    double cpx = (c > 0 && nr > 0) ? cpb[(size_t)c * ldc + k] : 0.0;
-    if (grp == 0)
-        for (int r = t; r < Ni; r += T) gq[r] = gny[r];
-    if (COEF)
-        for (int r = t; r < Ni; r += T) { shm.ca[r] = a_sub[r]; shm.cc[r] = c_sup[r]; }
-    if (grp == 0 || COEF) __syncthreads();  // uniform over the workgroup
+        for (int q = 0; q < NS; ++q) {
+            const int r = min(t + q * T, Ni - 1);
+            if (COEF) { sa[q] = a_sub[r]; sc[q] = c_sup[r]; }
+            if (SPLIT) sg[q] = gny[r];
+        }
+    }
+    const double kkx = kx2[kl];
+    const double cpx0 = cpb[(size_t)c * ldc + kl];
+    cplx f[RC];
+#pragma unroll
+    for (int i = 0; i < RC; ++i) f[i] = hat[(size_t)min(r0 + i + 1, Ni) * M + kl];
+    if (COEF || SPLIT) {
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+            const int r = t + q * T;
+            // rows Ni..NR-1 receive row Ni-1's values and are never read
+            if (NR % T == 0 || r < NR) {
+                if (COEF) { shm.ca[r] = sa[q]; shm.cc[r] = sc[q]; }
+                if (SPLIT) gq[r] = sg[q];
+            }
+        }
+        __syncthreads();  // uniform over the workgroup
+    }
     auto A = [&](int r) { return COEF ? shm.ca[r] : a_sub[r]; };
     auto C = [&](int r) { return COEF ? shm.cc[r] : c_sup[r]; };
+    double cpx = (c > 0 && nr > 0) ? cpx0 : 0.0;
+    double g[RC];
     cplx dpl = {0.0, 0.0}, S = {0.0, 0.0};
     double Wx = 1.0, Wy = 1.0, Px = 1.0, Py = 1.0, RPx = 0.0, RPy = 0.0;
 #pragma unroll
     for (int i = 0; i < RC; ++i) {
-        if (i < nr) {
-            const int r = r0 + i;
-            const double a = A(r), cc = C(r);
-            double rx = -f[i].x;
-            const double ry = -f[i].y;
-            if (split && r == 0) rx -= a * bc_bot;
-            if (split && r == Ni - 1) rx -= cc * bc_top;
-            const double gx = 1.0 / ((-(a + cc) - kkx) - a * cpx);
-            const double gy = split ? gq[r] : gx;
-            const double cpy = cc * gy;
-            cpx = cc * gx;
-            dpl = {(rx - a * dpl.x) * gx, (ry - a * dpl.y) * gy};
-            f[i] = dpl;
-            g[i] = gx;
-            Px *= -a * gx;
-            Py *= -a * gy;
-            S = {S.x + Wx * dpl.x, S.y + Wy * dpl.y};
-            RPx += Wx * Px;
-            RPy += Wy * Py;
-            Wx *= -cpx;
-            Wy *= -cpy;
-        }
+        // (a branch per row, as in the sweeps below: unguarded, the unrolled sweep holds more
+        // registers and spills at RC = 32)
+        if (i >= nr) continue;
+        const int r = r0 + i;
+        const double a = A(r), cc = C(r);
+        double rx = -f[i].x;
+        const double ry = -f[i].y;
+        if (split && r == 0) rx -= a * bc_bot;
+        if (split && r == Ni - 1) rx -= cc * bc_top;
+        const double gx = pivot_recip((-(a + cc) - kkx) - a * cpx);
+        const double gy = split ? gq[r] : gx;
+        cpx = cc * gx;
+        dpl = {(rx - a * dpl.x) * gx, (ry - a * dpl.y) * gy};
+        f[i] = dpl;
+        g[i] = gx;
+        Px *= -a * gx;
+        if (SPLIT) Py *= -a * gy;
+        S = {S.x + Wx * dpl.x, S.y + (SPLIT ? Wy : Wx) * dpl.y};
+        RPx += Wx * Px;
+        if (SPLIT) RPy += Wy * Py;
+        Wx *= -cpx;
+        if (SPLIT) Wy *= -(cc * gy);
     }
-    // forward carries: inclusive scan over chunks of the maps D -> E + P D (per component)
-    int buf = 0;
-    sm[0][0][t] = dpl.x;
-    sm[0][1][t] = dpl.y;
-    sm[0][2][t] = Px;  // nr == 0: the identity (0, 1)
-    sm[0][3][t] = Py;
-    __syncthreads();
+    if (!SPLIT) { Py = Px; Wy = Wx; RPy = RPx; }
+    // Chunk carries. Forward: D_c = dp at the end of chunk c-1, the composition over chunks
+    // 0..c-1 of the maps D -> E + P D (E = the chunk's local dp at its end). Backward: X_c =
+    // x at the start of chunk c+1, the composition over chunks c+1..NCH-1 of X -> h + Q X
+    // (h = S + RP D, Q = W). The imaginary part's P and Q are the real part's outside the
+    // slot-0 group. Chunk c of column kc sits in lane t = c*KC + kc, so the CPW = 64/KC chunks
+    // of one wavefront are composed with lane shuffles (no barrier); the wavefronts' totals
+    // then meet in LDS (one barrier per direction).
+    constexpr int CPW = 64 / KC, NW = T / 64;
+    const int lane = t & 63, w = t >> 6;
+    const bool first_in_wave = lane < KC, last_in_wave = lane >= 64 - KC;
+    double* agg = &sm[0][0][0];  // [4][NW][KC] wavefront totals (the scans' scratch)
+    {
+        double ex = dpl.x, ey = dpl.y, px = Px, py = Py;
 #pragma unroll
-    for (int d = 1; d < NCH; d <<= 1) {
-        double ex = sm[buf][0][t], ey = sm[buf][1][t], px = sm[buf][2][t], py = sm[buf][3][t];
-        if (c >= d) {
-            const int u = t - d * KC;
-            ex += px * sm[buf][0][u];
-            ey += py * sm[buf][1][u];
-            px *= sm[buf][2][u];
-            py *= sm[buf][3][u];
+        for (int d = 1; d < CPW; d <<= 1) {
+            const double ux = __shfl_up(ex, d * KC, 64), uy = __shfl_up(ey, d * KC, 64);
+            const double up = __shfl_up(px, d * KC, 64);
+            const double uq = SPLIT ? __shfl_up(py, d * KC, 64) : up;
+            if (lane >= d * KC) {
+                ex += px * ux;
+                ey += py * uy;
+                px *= up;
+                py = SPLIT ? py * uq : px;
+            }
         }
-        sm[buf ^ 1][0][t] = ex;
-        sm[buf ^ 1][1][t] = ey;
-        sm[buf ^ 1][2][t] = px;
-        sm[buf ^ 1][3][t] = py;
-        buf ^= 1;
+        if (last_in_wave) {
+            agg[(0 * NW + w) * KC + kc] = ex;
+            agg[(1 * NW + w) * KC + kc] = ey;
+            agg[(2 * NW + w) * KC + kc] = px;
+            agg[(3 * NW + w) * KC + kc] = py;
+        }
+        // exclusive within the wavefront: the inclusive value of the previous chunk
+        double dx = __shfl_up(ex, KC, 64), dy = __shfl_up(ey, KC, 64);
+        double qx = __shfl_up(px, KC, 64), qy = SPLIT ? __shfl_up(py, KC, 64) : qx;
+        if (first_in_wave) { dx = 0.0; dy = 0.0; qx = 1.0; qy = 1.0; }
         __syncthreads();
+        // carry into this wavefront: the earlier wavefronts' totals, composed in order
+        double cx = 0.0, cy = 0.0;
+        for (int v = 0; v < w; ++v) {
+            cx = agg[(0 * NW + v) * KC + kc] + agg[(2 * NW + v) * KC + kc] * cx;
+            cy = agg[(1 * NW + v) * KC + kc] + agg[(3 * NW + v) * KC + kc] * cy;
+        }
+        dpl = {dx + qx * cx, dy + qy * cy};  // now D of this chunk
     }
-    const cplx D = (c == 0) ? cplx{0.0, 0.0} : cplx{sm[buf][0][t - KC], sm[buf][1][t - KC]};
+    const cplx D = dpl;
     __syncthreads();
-    // backward carries: suffix scan of the maps X -> h + Q X, h = S + RP D
-    buf = 0;
-    sm[0][0][t] = S.x + RPx * D.x;
-    sm[0][1][t] = S.y + RPy * D.y;
-    sm[0][2][t] = Wx;
-    sm[0][3][t] = Wy;
-    __syncthreads();
+    {
+        double hx = S.x + RPx * D.x, hy = S.y + RPy * D.y, qx = Wx, qy = Wy;
 #pragma unroll
-    for (int d = 1; d < NCH; d <<= 1) {
-        double hx = sm[buf][0][t], hy = sm[buf][1][t], qx = sm[buf][2][t], qy = sm[buf][3][t];
-        if (c + d < NCH) {
-            const int u = t + d * KC;
-            hx += qx * sm[buf][0][u];
-            hy += qy * sm[buf][1][u];
-            qx *= sm[buf][2][u];
-            qy *= sm[buf][3][u];
+        for (int d = 1; d < CPW; d <<= 1) {
+            const double ux = __shfl_down(hx, d * KC, 64), uy = __shfl_down(hy, d * KC, 64);
+            const double uq = __shfl_down(qx, d * KC, 64);
+            const double ur = SPLIT ? __shfl_down(qy, d * KC, 64) : uq;
+            if (lane + d * KC < 64) {
+                hx += qx * ux;
+                hy += qy * uy;
+                qx *= uq;
+                qy = SPLIT ? qy * ur : qx;
+            }
         }
-        sm[buf ^ 1][0][t] = hx;
-        sm[buf ^ 1][1][t] = hy;
-        sm[buf ^ 1][2][t] = qx;
-        sm[buf ^ 1][3][t] = qy;
-        buf ^= 1;
+        if (first_in_wave) {
+            agg[(0 * NW + w) * KC + kc] = hx;
+            agg[(1 * NW + w) * KC + kc] = hy;
+            agg[(2 * NW + w) * KC + kc] = qx;
+            agg[(3 * NW + w) * KC + kc] = qy;
+        }
+        double xx = __shfl_down(hx, KC, 64), xy = __shfl_down(hy, KC, 64);
+        double rx = __shfl_down(qx, KC, 64), ry = SPLIT ? __shfl_down(qy, KC, 64) : rx;
+        if (last_in_wave) { xx = 0.0; xy = 0.0; rx = 1.0; ry = 1.0; }
         __syncthreads();
+        double cx = 0.0, cy = 0.0;
+        for (int v = NW - 1; v > w; --v) {
+            cx = agg[(0 * NW + v) * KC + kc] + agg[(2 * NW + v) * KC + kc] * cx;
+            cy = agg[(1 * NW + v) * KC + kc] + agg[(3 * NW + v) * KC + kc] * cy;
+        }
+        S = {xx + rx * cx, xy + ry * cy};  // now X of this chunk
     }
-    const cplx X = (c == NCH - 1) ? cplx{0.0, 0.0} : cplx{sm[buf][0][t + KC], sm[buf][1][t + KC]};
+    const cplx X = S;
     // true dp, then back substitution from x_{e+1} = X
     double Qx = 1.0, Qy = 1.0;
 #pragma unroll
     for (int i = 0; i < RC; ++i) {
         if (i < nr) {
-            const double a = A(r0 + i);
-            const double gy = split ? gq[r0 + i] : g[i];
+            const int r = r0 + i;
+            const double a = A(r);
             Qx *= -a * g[i];
-            Qy *= -a * gy;
-            f[i] = {f[i].x + Qx * D.x, f[i].y + Qy * D.y};
+            if (SPLIT) Qy *= -a * (split ? gq[r] : g[i]);
+            f[i] = {f[i].x + Qx * D.x, f[i].y + (SPLIT ? Qy : Qx) * D.y};
         }
     }
     cplx xn = X;
@@ -383,8 +451,11 @@ __global__ void __launch_bounds__(T) thomas_chunked_kernel(cplx* __restrict__ ha
                                                            double bc_top) {
     __shared__ ThomasShared<RC, KC, T> S;
     // column groups contiguous per XCD: groups narrower than a 128-byte line share lines in one L2
-    thomas_group<RC, KC, T>(S, xcd_tile(blockIdx.x, gridDim.x), hat, M, Ni, a_sub, c_sup, kx2, cpb, gny, bc_bot,
-                            bc_top);
+    const int grp = xcd_tile(blockIdx.x, gridDim.x);
+    if (grp == 0)
+        thomas_group<RC, KC, T, true>(S, grp, hat, M, Ni, a_sub, c_sup, kx2, cpb, gny, bc_bot, bc_top);
+    else
+        thomas_group<RC, KC, T, false>(S, grp, hat, M, Ni, a_sub, c_sup, kx2, cpb, gny, bc_bot, bc_top);
 }
 
 // A group of equally sized spectra sharing one plan: workgroup b solves column group
@@ -400,7 +471,11 @@ __global__ void __launch_bounds__(T) thomas_chunked_group_kernel(cplx* const* __
     __shared__ ThomasShared<RC, KC, T> S;
     const int b = xcd_tile(blockIdx.x, gridDim.x);
     const int m = b / groups;
-    thomas_group<RC, KC, T>(S, b - m * groups, hats[m], M, Ni, a_sub, c_sup, kx2, cpb, gny, bc_bot, bc_top);
+    const int grp = b - m * groups;
+    if (grp == 0)
+        thomas_group<RC, KC, T, true>(S, grp, hats[m], M, Ni, a_sub, c_sup, kx2, cpb, gny, bc_bot, bc_top);
+    else
+        thomas_group<RC, KC, T, false>(S, grp, hats[m], M, Ni, a_sub, c_sup, kx2, cpb, gny, bc_bot, bc_top);
 }
 
 int ilog2(int n) {
@@ -668,6 +743,14 @@ void poisson_tridiag(PoissonImpl* p, double psi_bot, double psi_top, hipStream_t
     NSE2D_LAUNCH_CHECK();
 }
 bool poisson_fuses_closure(const PoissonImpl* p) { return p->fft_radix != 0; }
+bool poisson_row_fft(const PoissonImpl* p, int* M, int* R, const void** twp, const void** tw) {
+    if (!p->fft_radix) return false;
+    *M = p->M;
+    *R = p->fft_radix;
+    *twp = p->d_twp;
+    *tw = p->d_tw;
+    return true;
+}
 
 bool poisson_inverse_closure(PoissonImpl* p, double* psi, double* omega, const double* dy_edge, double psi_bot,
                              double psi_top, int free_slip, hipStream_t s) {

@@ -358,3 +358,34 @@ def test_kernel_class_timing_and_sampling():
     assert np.array_equal(field(timed, nh.FIELD_OMEGA, g), field(plain, nh.FIELD_OMEGA, g))
     for ch in chans:
         L.nse2d_channel_destroy(ch)
+
+
+@pytest.mark.parametrize("Nx,Ny", [(512, 257), (1024, 129), (2048, 97), (4096, 65), (8192, 35), (16384, 19)])
+def test_fused_stage_transform_equals_separate_kernels(Nx, Ny, monkeypatch):
+    """With NSE2D_STAGE_FFT=1 (at creation) production stages run as one kernel with the forward
+    row transform where the row-FFT plan allows it; by default the stage kernel and the transform
+    run separately. Both give the same fields, dt and time bit for bit over adaptive steps, on
+    every instantiated plan (M = Nx/2 from 256 to 8192)."""
+    print(f"[stage+fft] {Nx}x{Ny}", flush=True)
+    g = Grid(Nx, Ny, 2.0)
+    w0 = channel_ic(Nx, Ny)
+    chans = []
+    for flag in ("0", "1"):
+        monkeypatch.setenv("NSE2D_STAGE_FFT", flag)
+        ch = make_channel(g, 2e-4, 0.05, 12, 1e-3, 2.0)
+        nh.upload_ptr(nh.lib().nse2d_channel_field(ch, nh.FIELD_OMEGA), w0)
+        nh.lib().nse2d_channel_init(ch)
+        chans.append(ch)
+    try:
+        for ch in chans:
+            nh.lib().nse2d_channel_step_auto(ch, 6)
+        sync()
+        (a, pa), (b, pb) = ((field(ch, nh.FIELD_OMEGA, g), field(ch, nh.FIELD_PSI, g)) for ch in chans)
+        ta, tb = (nh.download_ptr(nh.lib().nse2d_channel_time_ptr(ch), 2) for ch in chans)
+        print(f"[stage+fft] {Nx}x{Ny}: max |omega| diff {np.max(np.abs(a - b)):.3e}, "
+              f"max |psi| diff {np.max(np.abs(pa - pb)):.3e}", flush=True)
+        assert np.all(np.isfinite(a))
+        assert np.array_equal(a, b) and np.array_equal(pa, pb) and np.array_equal(ta, tb)
+    finally:
+        for ch in chans:
+            nh.lib().nse2d_channel_destroy(ch)
