@@ -415,7 +415,9 @@ const char* nse2d_last_error(void);
 const char* nse2d_version(void);
 
 /* Host-side setup with the reference's definitions: grid metrics (grid.cu:7-94; stretch "none",
- * "uniform" or "tanh") and the initial vorticity (init.cu:99-378, plus the seeded benchmark ICs
+ * "uniform", "tanh", or "table:<file>" for the tabulated grid of grid.cu:46-78 with the Ny values
+ * of y read as whitespace-separated text in place of the reference's HDF5 dataset /y) and the
+ * initial vorticity (init.cu:99-378, plus the seeded benchmark ICs
  * "lamnoise" and "lamband"); argv is the reference command line. Return 0 on success. */
 int nse2d_host_metrics(int Ny, double h, const char* stretch, double beta, double* y,
                        double* a_node, double* a_edge, double* dy_edge, double* w_node);

@@ -914,6 +914,10 @@ extern "C" int nse2d_host_metrics(int Ny, double h, const char* stretch, double 
     G.h = h;
     Params P;
     P.stretch = stretch ? stretch : "none";
+    if (P.stretch.compare(0, 6, "table:") == 0) {  // "table:<file>": --stretch table --ytable <file>
+        P.ytable = P.stretch.substr(6);
+        P.stretch = "table";
+    }
     P.beta = beta;
     std::string err;
     if (Ny < 3 || !build_y_metrics(G, P, err)) {

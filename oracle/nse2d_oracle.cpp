@@ -51,6 +51,19 @@ double row_sum(int j0, int j1, F f) {
 // =============================================================================================
 // grid.cu:7-94
 // =============================================================================================
+namespace {
+// edges, spacings, trapezoid weights, grid.cu:83-93
+void edges_and_weights(int Ny, const double* y, const double* a_node, double* a_edge, double* dy_edge,
+                       double* w_node) {
+    for (int j = 0; j < Ny - 1; ++j) a_edge[j] = 0.5 * (a_node[j] + a_node[j + 1]);
+    for (int j = 0; j < Ny - 1; ++j) dy_edge[j] = y[j + 1] - y[j];
+    for (int j = 0; j < Ny; ++j) w_node[j] = 0.0;
+    w_node[0] = 0.5 * dy_edge[0];
+    w_node[Ny - 1] = 0.5 * dy_edge[Ny - 2];
+    for (int j = 1; j < Ny - 1; ++j) w_node[j] = 0.5 * (dy_edge[j - 1] + dy_edge[j]);
+}
+}  // namespace
+
 extern "C" int orc_build_y_metrics(int Ny, double h, int stretch, double beta, double* y,
                                    double* a_node, double* a_edge, double* dy_edge,
                                    double* w_node) {
@@ -67,7 +80,7 @@ extern "C" int orc_build_y_metrics(int Ny, double h, int stretch, double beta, d
         w_node[0] *= 0.5; w_node[Ny - 1] *= 0.5;
         return 0;
     }
-    if (stretch != 1) return -2;  // tabulated grids need the reference's HDF5 reader
+    if (stretch != 1) return -2;  // tabulated grids: orc_table_y_metrics
     // tanh branch, grid.cu:33-45
     const double b = (beta > 0 ? beta : 2.5);
     for (int j = 0; j < Ny; ++j) {
@@ -80,13 +93,26 @@ extern "C" int orc_build_y_metrics(int Ny, double h, int stretch, double beta, d
         const double y_eta = h * b * (sech * sech) / std::tanh(b);
         a_node[j] = 1.0 / y_eta;
     }
-    // edges, spacings, trapezoid weights, grid.cu:83-93
-    for (int j = 0; j < Ny - 1; ++j) a_edge[j] = 0.5 * (a_node[j] + a_node[j + 1]);
-    for (int j = 0; j < Ny - 1; ++j) dy_edge[j] = y[j + 1] - y[j];
-    for (int j = 0; j < Ny; ++j) w_node[j] = 0.0;
-    w_node[0] = 0.5 * dy_edge[0];
-    w_node[Ny - 1] = 0.5 * dy_edge[Ny - 2];
-    for (int j = 1; j < Ny - 1; ++j) w_node[j] = 0.5 * (dy_edge[j - 1] + dy_edge[j]);
+    edges_and_weights(Ny, y, a_node, a_edge, dy_edge, w_node);
+    return 0;
+}
+
+// Tabulated branch, grid.cu:46-78: y given (the reference reads dataset /y from an HDF5 file;
+// the caller passes the Ny values), a_node = 1/y_eta with y_eta from centred differences in eta
+// and one-sided ones at the two walls.
+extern "C" int orc_table_y_metrics(int Ny, const double* y_in, double* y, double* a_node, double* a_edge,
+                                   double* dy_edge, double* w_node) {
+    if (Ny < 3) return -1;
+    for (int j = 0; j < Ny; ++j) y[j] = y_in[j];
+    const double deta = 2.0 / (double)(Ny - 1);
+    for (int j = 0; j < Ny; ++j) {
+        double yeta;
+        if (j == 0) yeta = (y[1] - y[0]) / deta;
+        else if (j == Ny - 1) yeta = (y[Ny - 1] - y[Ny - 2]) / deta;
+        else yeta = (y[j + 1] - y[j - 1]) / (2.0 * deta);
+        a_node[j] = 1.0 / yeta;
+    }
+    edges_and_weights(Ny, y, a_node, a_edge, dy_edge, w_node);
     return 0;
 }
 
@@ -973,7 +999,7 @@ struct OGrid {  // common.cuh:42-57
 
 struct OParams {  // common.cuh:58-114
     double Ub{1.0};
-    std::string stretch{"none"};
+    std::string stretch{"none"}, ytable;
     double beta{0.0};
     double Reb{10000.0};
     double nu_override{-1};
@@ -1039,7 +1065,7 @@ void parse_args(int argc, char** argv, ORun& R) {
         else if (eq(argv[i], "--nu") && nx) P.nu_override = std::atof(argv[++i]);
         else if (eq(argv[i], "--stretch") && nx) P.stretch = argv[++i];
         else if (eq(argv[i], "--beta") && nx) P.beta = std::atof(argv[++i]);
-        else if (eq(argv[i], "--ytable") && nx) ++i;
+        else if (eq(argv[i], "--ytable") && nx) P.ytable = argv[++i];
         else if (eq(argv[i], "--integrator") && nx) ++i;
         else if (eq(argv[i], "--init") && nx) P.init = argv[++i];
         else if (eq(argv[i], "--amp") && nx) P.amp = std::atof(argv[++i]);
@@ -1094,6 +1120,15 @@ bool build_grid(ORun& R) {
     int st;
     if (R.P.stretch == "none" || R.P.stretch == "uniform") st = 0;
     else if (R.P.stretch == "tanh") st = 1;
+    else if (R.P.stretch == "table" && Ny > 2) {
+        // the Ny values of y as whitespace-separated text (in place of the reference's HDF5 /y)
+        std::ifstream f(R.P.ytable);
+        std::vector<double> yt((size_t)Ny);
+        for (int j = 0; j < Ny; ++j)
+            if (!(f >> yt[(size_t)j])) { std::fprintf(stderr, "oracle: --ytable needs %d values\n", Ny); return false; }
+        return orc_table_y_metrics(Ny, yt.data(), G.y.data(), G.a_node.data(), G.a_edge.data(), G.dy_edge.data(),
+                                   G.w_node.data()) == 0;
+    }
     else { std::fprintf(stderr, "oracle: unsupported --stretch %s\n", R.P.stretch.c_str()); return false; }
     return orc_build_y_metrics(Ny, G.h, st, R.P.beta, G.y.data(), G.a_node.data(), G.a_edge.data(),
                                G.dy_edge.data(), G.w_node.data()) == 0;

@@ -24,6 +24,9 @@ extern "C" {
 // ---- grid.cu:7-94 build_y_metrics. stretch: 0 = none/uniform, 1 = tanh. Returns 0 on success.
 int orc_build_y_metrics(int Ny, double h, int stretch, double beta,
                         double* y, double* a_node, double* a_edge, double* dy_edge, double* w_node);
+// ---- grid.cu:46-78 tabulated branch: y given (Ny values), metrics by differences in eta.
+int orc_table_y_metrics(int Ny, const double* y_in, double* y, double* a_node, double* a_edge,
+                        double* dy_edge, double* w_node);
 
 // ---- ops.cu (device operators restated as loops)
 void orc_uv_from_psi_mapped(const double* psi, double* u, double* v, int Nx, int Ny,

@@ -13,6 +13,7 @@ EXE_PATH = os.path.join(ORACLE_DIR, "oracle_nse2d")
 _d, _i, _sz, _vp = ctypes.c_double, ctypes.c_int, ctypes.c_size_t, ctypes.c_void_p
 SIG = {
     "orc_build_y_metrics": (_i, [_i, _d, _i, _d, _vp, _vp, _vp, _vp, _vp]),
+    "orc_table_y_metrics": (_i, [_i, _vp, _vp, _vp, _vp, _vp, _vp]),
     "orc_uv_from_psi_mapped": (None, [_vp, _vp, _vp, _i, _i, _d, _d, _vp]),
     "orc_omega_from_psi_interior_mapped": (None, [_vp, _vp, _i, _i, _d, _d, _vp, _vp]),
     "orc_thom_wall_vorticity_mapped": (None, [_vp, _vp, _i, _i, _vp, _d, _d, _i]),
@@ -98,6 +99,17 @@ class Metrics:
         assert rc == 0
         self.wt_node = np.zeros(Ny)
         self.wt_node[1:-1] = (2.0 / (Ny - 1)) / self.a_node[1:-1]
+
+
+def table_metrics(y):
+    """Metrics of a tabulated grid (grid.cu:46-78) from its Ny values of y, by the oracle:
+    (y, a_node, a_edge, dy_edge, w_node)."""
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    Ny = len(y)
+    out = [np.zeros(Ny), np.zeros(Ny), np.zeros(Ny - 1), np.zeros(Ny - 1), np.zeros(Ny)]
+    rc = lib().orc_table_y_metrics(Ny, P(y), *[P(a) for a in out])
+    assert rc == 0, rc
+    return out
 
 
 def init_omega(args, Nx, Ny):
