@@ -179,7 +179,10 @@ def run_ensemble(a):
         nh.upload_ptr(L.nse2d_ensemble_field(e, k, nh.FIELD_OMEGA), w0)
     L.nse2d_ensemble_init(e)
     rep = ctypes.create_string_buffer(1024)
-    L.nse2d_ensemble_autotune(e, 3, rep, 1024)
+    if a.no_autotune:
+        rep.value = b"off (shapes from NSE2D_STAGE_TILE / NSE2D_FFT_RADIX / NSE2D_THOMAS_KC or the defaults)"
+    else:
+        L.nse2d_ensemble_autotune(e, 3, rep, 1024)
     L.nse2d_ensemble_step_auto(e, a.warmup)
     L.nse2d_ensemble_sync(e)
     t0 = time.perf_counter()
@@ -295,6 +298,7 @@ def main():
     ap.add_argument("--time-every", type=int, default=10,
                     help="in the timed region, bracket the dominant kernel with events on every k-th step")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="ensemble: keep the configured kernel shapes")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first run 5 device-to-device copies of 256 MiB (a plain copy kernel of known size, "
                          "for scaling HBM counter readings)")

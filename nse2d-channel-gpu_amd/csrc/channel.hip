@@ -402,6 +402,25 @@ __global__ void __launch_bounds__(64 * SYW) stage_kernel_wave2_group(const Stage
     stage_wave2_body<STAGE, RY, SYW, PF>(args[m], b - tile_begin[m]);
 }
 
+// Persistent form of the grouped two-column stage: a fixed grid (a few workgroups per CU) walks
+// all members' tiles. Each XCD owns the same contiguous range of global tiles as under xcd_tile,
+// and its workgroups take that range in order, gx tiles apart, so that neighbouring tiles are in
+// flight on one XCD at the same time.
+template <int STAGE, int RY, int SYW, int PF>
+__global__ void __launch_bounds__(64 * SYW) stage_kernel_wave2_group_persistent(const StageArgs* __restrict__ args,
+                                                                                const int* __restrict__ tile_begin,
+                                                                                int n, int total) {
+    const int x = blockIdx.x % kXcds, l = blockIdx.x / kXcds, G = gridDim.x;
+    const int gx = G / kXcds + (x < G % kXcds ? 1 : 0);  // workgroups on this XCD
+    const int q = total / kXcds, rr = total % kXcds;
+    const int t0 = x * q + min(x, rr), t1 = t0 + q + (x < rr ? 1 : 0);
+    int m = 0;
+    for (int b = t0 + l; b < t1; b += gx) {
+        while (m + 1 < n && tile_begin[m + 1] <= b) ++m;  // b only grows: the member index only advances
+        stage_wave2_body<STAGE, RY, SYW, PF>(args[m], b - tile_begin[m]);
+    }
+}
+
 // Forcing profile f(y_j) (ops.cu:526-531), evaluated once per row.
 __global__ void forcing_rows_kernel(double* f, const double* y, int Ny, double F0, int nforce, double h) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
@@ -643,7 +662,9 @@ int stage_tile_code() {
     int ry = 8, sy = 2, pf = 1, wide = 1;
     if (const char* e = std::getenv("NSE2D_STAGE_TILE")) {
         if (e[0] == '0') return 0;
+        wide = 0;  // one column per lane unless the shape says otherwise
         if (e[0] == 'w') { wide = 1; ++e; }
+        else if (e[0] == 'p') { wide = 11; ++e; }  // ensembles: the persistent group kernel
         const int n = std::sscanf(e, "%dx%dx%d", &ry, &sy, &pf);
         if (n == 2) pf = 2;
         else if (n != 3) return kStageDefaultWide;
@@ -652,7 +673,7 @@ int stage_tile_code() {
     switch (c) {
         case 841: case 842: case 843: case 1641: case 1642: case 1643: case 1621: case 1622: case 1623:
         case 3211: case 3212: case 3213:
-        case 10841: case 10842: case 10441: case 11621: case 10821: return c;
+        case 10841: case 10842: case 10441: case 11621: case 10821: case 110821: case 110841: return c;
         default: return kStageDefaultWide;
     }
 }
@@ -815,8 +836,10 @@ struct EnsMember {
 int stage_code_ry(int c) { return (c % 10000) / 100; }
 int stage_code_sy(int c) { return (c / 10) % 10; }
 bool stage_code_wide(int c) { return c >= 10000; }
+bool stage_code_persistent(int c) { return c >= 100000; }  // ensembles: the persistent group kernel
 std::string stage_code_name(int c) {
-    return (stage_code_wide(c) ? "w" : "") + std::to_string(stage_code_ry(c)) + "x" + std::to_string(stage_code_sy(c)) +
+    return (stage_code_persistent(c) ? "p" : stage_code_wide(c) ? "w" : "") + std::to_string(stage_code_ry(c)) + "x" +
+           std::to_string(stage_code_sy(c)) +
            "x" + std::to_string(c % 10);
 }
 
@@ -840,6 +863,7 @@ struct EnsembleImpl {
     bool use_graph = true;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    int cus = 256;  // compute units of the device (grid of the persistent stage kernel)
 };
 
 namespace {
@@ -875,6 +899,17 @@ void ens_launch_stage(EnsembleImpl* e, int stage) {
         else                                                                                               \
             stage_kernel_wave_group<3, RYV, SYV, PFV><<<e->total_tiles, dim3(64, SYV), 0, e->s>>>(args, e->d_tile_begin, n); \
         break;
+#define NSE2D_ENS_PERSIST(RYV, SYV, PFV)                                                                   \
+    case 110000 + RYV * 100 + SYV * 10 + PFV: {                                                            \
+        const int g = std::min(e->total_tiles, e->cus * (16 / SYV));                                       \
+        if (stage == 1)                                                                                    \
+            stage_kernel_wave2_group_persistent<1, RYV, SYV, PFV><<<g, dim3(64, SYV), 0, e->s>>>(args, e->d_tile_begin, n, e->total_tiles); \
+        else if (stage == 2)                                                                               \
+            stage_kernel_wave2_group_persistent<2, RYV, SYV, PFV><<<g, dim3(64, SYV), 0, e->s>>>(args, e->d_tile_begin, n, e->total_tiles); \
+        else                                                                                               \
+            stage_kernel_wave2_group_persistent<3, RYV, SYV, PFV><<<g, dim3(64, SYV), 0, e->s>>>(args, e->d_tile_begin, n, e->total_tiles); \
+        break;                                                                                             \
+    }
 #define NSE2D_ENS_WAVE2(RYV, SYV, PFV)                                                                     \
     case 10000 + RYV * 100 + SYV * 10 + PFV:                                                               \
         if (stage == 1)                                                                                    \
@@ -891,10 +926,12 @@ void ens_launch_stage(EnsembleImpl* e, int stage) {
         NSE2D_ENS_WAVE(32, 1, 1) NSE2D_ENS_WAVE(32, 1, 2) NSE2D_ENS_WAVE(32, 1, 3)
         NSE2D_ENS_WAVE2(8, 4, 1) NSE2D_ENS_WAVE2(8, 4, 2) NSE2D_ENS_WAVE2(4, 4, 1)
         NSE2D_ENS_WAVE2(16, 2, 1) NSE2D_ENS_WAVE2(8, 2, 1)
+        NSE2D_ENS_PERSIST(8, 2, 1) NSE2D_ENS_PERSIST(8, 4, 1)
         default: std::abort();
     }
 #undef NSE2D_ENS_WAVE
 #undef NSE2D_ENS_WAVE2
+#undef NSE2D_ENS_PERSIST
     NSE2D_LAUNCH_CHECK();
 }
 
@@ -1078,6 +1115,12 @@ EnsembleImpl* ens_create(const nse2d_channel_desc* descs, int n, hipStream_t s) 
     NSE2D_HIP_CHECK(hipMemcpyAsync(e->d_jobs, jobs.data(), sizeof(MaxUVJob) * n, hipMemcpyHostToDevice, e->s));
     e->all_wide = true;
     for (const EnsMember& M : e->mem) e->all_wide = e->all_wide && (M.d.Nx % 128 == 0);
+    {
+        int dev = 0, cus = 0;
+        NSE2D_HIP_CHECK(hipGetDevice(&dev));
+        NSE2D_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        if (cus > 0) e->cus = cus;
+    }
     e->stage_tile = stage_tile_code();
     // the generic kernel has no group form; the two-column kernels need every Nx % 128 == 0
     if (e->stage_tile == 0 || (stage_code_wide(e->stage_tile) && !e->all_wide)) e->stage_tile = kStageDefault;
@@ -1135,7 +1178,7 @@ std::string ens_autotune(EnsembleImpl* e, int nsteps) {
     std::string rep;
     {
         const int codes[] = {841,   842,   843,   1641,  1642,  1643,  1621, 1622, 1623, 3211, 3212, 3213,
-                             10821, 10841, 10842, 10441, 11621};
+                             10821, 10841, 10842, 10441, 11621, 110821, 110841};
         int best = e->stage_tile;
         double tbest = 1e300;
         for (int code : codes) {
@@ -1220,6 +1263,7 @@ nse2d_channel_t nse2d_channel_create(const nse2d_channel_desc* desc, hipStream_t
     c->s = s;
     c->P = P;
     c->stage_tile = stage_tile_code();
+    if (c->stage_tile >= 100000) c->stage_tile -= 100000;  // a single channel has no persistent form
     if (const char* e = std::getenv("NSE2D_STAGE_FFT")) c->stage_fft = (e[0] == '1');
     c->dx = d.Lx / (double)d.Nx;
     c->inv_dx = 1.0 / c->dx;

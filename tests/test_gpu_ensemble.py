@@ -202,3 +202,24 @@ def test_rejects_unsupported_members():
     arr = (nh.ChannelDesc * 1)(bad)
     assert not nh.lib().nse2d_ensemble_create(arr, 1, None)
     assert "manufactured" in nh.last_error()
+
+
+@pytest.mark.parametrize("tile", ["p8x2x1", "p8x4x1"])
+def test_persistent_stage_equals_one_tile_per_workgroup(tile, monkeypatch):
+    """The persistent grouped stage kernel (a fixed grid walking every member's tiles; selected by
+    NSE2D_STAGE_TILE=p... or by the autotuner) advances the ensemble exactly as the grouped kernel
+    with one workgroup per tile."""
+    print(f"[ensemble] {tile} vs w{tile[1:]}", flush=True)
+    S = Setup()
+    runs = []
+    for t in (tile, "w" + tile[1:]):
+        monkeypatch.setenv("NSE2D_STAGE_TILE", t)
+        e = S.ensemble()
+        nh.lib().nse2d_ensemble_step_auto(e, 10)
+        nh.lib().nse2d_ensemble_sync(e)
+        runs.append(([S.fields(e, m) for m in range(len(S.members))], S.scalars(e)))
+        nh.lib().nse2d_ensemble_destroy(e)
+    (fa, sa), (fb, sb) = runs
+    for m in range(len(S.members)):
+        assert np.array_equal(fa[m][0], fb[m][0]) and np.array_equal(fa[m][1], fb[m][1]), m
+    assert np.array_equal(sa, sb)
