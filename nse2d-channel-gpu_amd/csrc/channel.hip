@@ -533,15 +533,26 @@ __global__ void __launch_bounds__(kCflThreads) cfl_rows2_kernel(const double* __
         for (int k = 0; k < 3; ++k) partial[(size_t)k * gridDim.x + b] = m[k];
 }
 
-__global__ void __launch_bounds__(kCflThreads) cfl_final_kernel(const double* __restrict__ partial, int Ny,
-                                                                double* __restrict__ scal, DtRule r) {
+// One workgroup of 1024 threads; each pass issues four loads per maximum before reducing them,
+// so the n partials (1024 at 4096x2049) cost one memory latency instead of one per load.
+constexpr int kCflFinalThreads = 1024;
+__global__ void __launch_bounds__(kCflFinalThreads) cfl_final_kernel(const double* __restrict__ partial, int n,
+                                                                     double* __restrict__ scal, DtRule r) {
     __shared__ double sm[3 * 16];
-    double m[3];
+    double m[3] = {0.0, 0.0, 0.0};
+    for (int i0 = 0; i0 < n; i0 += 4 * kCflFinalThreads) {
+        double v[3][4];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        double v = 0.0;
-        for (int i = threadIdx.x; i < Ny; i += kCflThreads) v = fmax(v, partial[(size_t)a * Ny + i]);
-        m[a] = v;
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int i = i0 + u * kCflFinalThreads + (int)threadIdx.x;
+                v[a][u] = (i < n) ? partial[(size_t)a * n + i] : 0.0;
+            }
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) m[a] = fmax(m[a], v[a][u]);
     }
     block_max<3>(m, sm);
     if (threadIdx.x == 0) {
@@ -1357,7 +1368,7 @@ void nse2d_channel_step_auto(nse2d_channel_t ch, int n) {
                                                                     d.d_a_node, c->cfl_partial);
                 }
                 NSE2D_LAUNCH_CHECK();
-                cfl_final_kernel<<<1, kCflThreads, 0, c->s>>>(c->cfl_partial, nparts, c->scalars, rule);
+                cfl_final_kernel<<<1, kCflFinalThreads, 0, c->s>>>(c->cfl_partial, nparts, c->scalars, rule);
             }
             NSE2D_LAUNCH_CHECK();
         }
