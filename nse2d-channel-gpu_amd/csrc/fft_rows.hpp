@@ -270,16 +270,29 @@ __device__ __forceinline__ void inverse_row(const cplx* __restrict__ src, cplx (
                                    const cplx* __restrict__ twp, const cplx* __restrict__ tw, double scale) {
     using P = Plan<M, R>;
     constexpr int T = P::T;
+    constexpr int J = (M / 2 + 1 + T - 1) / T;  // slots k = tid + j T of the half spectrum
     const int tid = threadIdx.x;
-    for (int k = tid; k <= M / 2; k += T) {
+    // all loads first (clamped to a valid k, k = 0 pairing with itself), so that a row costs one
+    // memory latency instead of one per slot
+    cplx Xk[J], Xm[J], Wk[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const int k = min(tid + j * T, M / 2);
+        Xk[j] = src[k];
+        Xm[j] = src[k == 0 ? 0 : M - k];
+        Wk[j] = tw[k];
+    }
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const int k = tid + j * T;
+        if (k > M / 2) break;  // only the last slot is partial
         if (k == 0) {
-            const cplx p = src[0];  // (X[0], X[M])
+            const cplx p = Xk[0];  // (X[0], X[M])
             lds[0] = {p.x + p.y, p.x - p.y};
             continue;
         }
-        const cplx Xk = src[k], Xm = src[M - k];
-        const cplx A = cadd(Xk, cconj(Xm)), B = csub(Xk, cconj(Xm));
-        const cplx WB = cmulc(B, tw[k]);          // conj(W^k) B
+        const cplx A = cadd(Xk[j], cconj(Xm[j])), B = csub(Xk[j], cconj(Xm[j]));
+        const cplx WB = cmulc(B, Wk[j]);          // conj(W^k) B
         const cplx iWB = {-WB.y, WB.x};
         lds[lds_pad(k)] = cadd(A, iWB);
         if (k != M - k) lds[lds_pad(M - k)] = cconj(csub(A, iWB));
