@@ -262,19 +262,35 @@ __device__ __forceinline__ double2 stage_pair(const StageArgs& a, int j, int ix,
     return double2{o[0], o[1]};
 }
 
-template <int STAGE, int RY, int SYW, int PF>
+// ROWS_FIRST: the row factors of all RY rows are loaded at the start (scalar loads into SGPRs
+// when the arguments are kernel arguments; the group kernels, whose arguments come from device
+// memory, would hold them in VGPRs and keep the row-by-row loads).
+template <int STAGE, int RY, int SYW, int PF, bool ROWS_FIRST = true>
 __device__ __forceinline__ void stage_wave2_body(const StageArgs& a, int tile) {
     const int Nx = a.Nx, Ny = a.Ny;
     const int lane = threadIdx.x;
     const int ntx = Nx >> 7;
     const int x0 = (tile % ntx) << 7, ix = x0 + 2 * lane;
-    const int jlo = ((tile / ntx) * SYW + threadIdx.y) * RY;
+    // the wavefront's first row (uniform over the wavefront: its row factors are scalar loads)
+    const int jlo = __builtin_amdgcn_readfirstlane(((tile / ntx) * SYW + (int)threadIdx.y) * RY);
     if (jlo >= Ny) return;  // uniform over the wavefront
     const int jhi = min(jlo + RY, Ny);
     const bool edge = (lane == 0) || (lane == 63);
     const int hx = (lane == 0) ? (x0 == 0 ? Nx - 1 : x0 - 1) : (x0 + 128 == Nx ? 0 : x0 + 128);
     const double inv4dxde = 0.25 * a.inv_dx * a.inv_deta;
     const double dt = a.dt_ptr ? *a.dt_ptr : a.dt_val;
+    // The row factors of all RY rows, loaded first. Loaded row by row, after the previous row's
+    // store (which the compiler cannot tell apart from them), each would wait behind the row
+    // prefetched ahead of it, and every row would pay a full memory latency.
+    RowCoef rk[ROWS_FIRST ? RY : 1];
+    if constexpr (ROWS_FIRST) {
+#pragma unroll
+        for (int q = 0; q < RY; ++q) rk[q] = row_coef(a, min(jlo + q, Ny - 1));
+    }
+    auto row_factors = [&](int q, int j) {
+        if constexpr (ROWS_FIRST) return rk[q];
+        else return row_coef(a, j);
+    };
     Row4 Pm{}, Pc{}, Pp{}, Wm{}, Wc{}, Wp{};
     PairRow ring[PF];
     double2 wring[PF];
@@ -305,7 +321,7 @@ __device__ __forceinline__ void stage_wave2_body(const StageArgs& a, int tile) {
             wring[q % PF] = *reinterpret_cast<const double2*>(a.wn + idx2(j + PF, ix, Nx));
         if (j + 1 < Ny) expand_pair_row(cur, Pp, Wp);
         *reinterpret_cast<double2*>(a.out + idx2(j, ix, Nx)) =
-            stage_pair<STAGE>(a, j, ix, Pm, Pc, Pp, Wm, Wc, Wp, wn_c, dt, inv4dxde, row_coef(a, j), a.rhs_out);
+            stage_pair<STAGE>(a, j, ix, Pm, Pc, Pp, Wm, Wc, Wp, wn_c, dt, inv4dxde, row_factors(q, j), a.rhs_out);
         Pm = Pc; Pc = Pp; Wm = Wc; Wc = Wp;
     }
 }
@@ -399,7 +415,7 @@ __global__ void __launch_bounds__(64 * SYW) stage_kernel_wave2_group(const Stage
     const int b = xcd_tile(blockIdx.x, gridDim.x);
     int m = 0;
     while (m + 1 < n && tile_begin[m + 1] <= b) ++m;
-    stage_wave2_body<STAGE, RY, SYW, PF>(args[m], b - tile_begin[m]);
+    stage_wave2_body<STAGE, RY, SYW, PF, false>(args[m], b - tile_begin[m]);
 }
 
 // Persistent form of the grouped two-column stage: a fixed grid (a few workgroups per CU) walks
@@ -417,7 +433,7 @@ __global__ void __launch_bounds__(64 * SYW) stage_kernel_wave2_group_persistent(
     int m = 0;
     for (int b = t0 + l; b < t1; b += gx) {
         while (m + 1 < n && tile_begin[m + 1] <= b) ++m;  // b only grows: the member index only advances
-        stage_wave2_body<STAGE, RY, SYW, PF>(args[m], b - tile_begin[m]);
+        stage_wave2_body<STAGE, RY, SYW, PF, false>(args[m], b - tile_begin[m]);
     }
 }
 

@@ -196,7 +196,16 @@ __device__ __forceinline__ void forward_row_from(cplx (&v)[R], cplx* __restrict_
     for (int q = 0; q < R; ++q) lds[lds_pad(tid * R + q)] = v[q];
     __syncthreads();
     lds_passes<M, R, false>(v, lds, twp, 1.0, std::make_integer_sequence<int, P::NPASS - 1>{});
-    for (int k = tid; k <= M / 2; k += T) {
+    // the split factors of this thread's slots k = tid + j T, loaded together (clamped to
+    // k <= M/2): one memory latency instead of one per slot
+    constexpr int J = (M / 2 + 1 + T - 1) / T;
+    cplx Wk[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) Wk[j] = tw[min(tid + j * T, M / 2)];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const int k = tid + j * T;
+        if (k > M / 2) break;  // only the last slot is partial
         const cplx Zk = lds[lds_pad(k)], Zm = lds[lds_pad((M - k) & (M - 1))];
         if (k == 0) {
             dst[0] = {Zk.x + Zk.y, Zk.x - Zk.y};  // (X[0], X[M])
@@ -205,7 +214,7 @@ __device__ __forceinline__ void forward_row_from(cplx (&v)[R], cplx* __restrict_
         const cplx E = cscale(cadd(Zk, cconj(Zm)), 0.5);
         const cplx D = csub(Zk, cconj(Zm));
         const cplx O = {0.5 * D.y, -0.5 * D.x};  // -i/2 * D
-        const cplx WO = cmul(tw[k], O);
+        const cplx WO = cmul(Wk[j], O);
         dst[k] = cadd(E, WO);
         if (k != M - k) dst[M - k] = cconj(csub(E, WO));
     }

