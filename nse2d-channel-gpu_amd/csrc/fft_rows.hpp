@@ -134,6 +134,17 @@ __device__ __forceinline__ void lds_pass(cplx (&v)[R], cplx* lds, const cplx* __
     constexpr int T = P::T, Rp = P::radix(p), Ns = P::ns(p), NI = R / Rp, MR = M / Rp;
     constexpr bool LAST = (p == P::NPASS - 1);
     const int tid = threadIdx.x;
+    // the pass's twiddles first: they do not depend on the data, so their memory latency runs
+    // under the LDS reads and the barrier instead of after them
+    cplx wv[NI][Rp > 1 ? Rp - 1 : 1];
+    if constexpr (Ns > 1) {
+#pragma unroll
+        for (int i = 0; i < NI; ++i) {
+            const cplx* w = twp + P::twoff(p) + ((tid + i * T) & (Ns - 1)) * (Rp - 1);
+#pragma unroll
+            for (int q = 1; q < Rp; ++q) wv[i][q - 1] = w[q - 1];
+        }
+    }
 #pragma unroll
     for (int i = 0; i < NI; ++i)
 #pragma unroll
@@ -143,9 +154,9 @@ __device__ __forceinline__ void lds_pass(cplx (&v)[R], cplx* lds, const cplx* __
     for (int i = 0; i < NI; ++i) {
         const int j = tid + i * T, k = j & (Ns - 1);
         if constexpr (Ns > 1) {
-            const cplx* w = twp + P::twoff(p) + k * (Rp - 1);
 #pragma unroll
-            for (int q = 1; q < Rp; ++q) v[i * Rp + q] = INV ? cmulc(v[i * Rp + q], w[q - 1]) : cmul(v[i * Rp + q], w[q - 1]);
+            for (int q = 1; q < Rp; ++q)
+                v[i * Rp + q] = INV ? cmulc(v[i * Rp + q], wv[i][q - 1]) : cmul(v[i * Rp + q], wv[i][q - 1]);
         }
         dft<Rp, INV>(v + i * Rp);
         const int base = (j - k) * Rp + k;
