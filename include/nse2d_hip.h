@@ -268,7 +268,7 @@ enum { NSE2D_PROF_STAGE = 0, NSE2D_PROF_FFT_FWD = 1, NSE2D_PROF_THOMAS = 2, NSE2
        NSE2D_PROF_THOM = 4, NSE2D_PROF_CFL = 5, NSE2D_PROF_NCLASS = 6 };
 void nse2d_channel_profile(nse2d_channel_t ch, int class_mask);
 /* Time only the first of every `every` steps (default 1: every step). The events themselves
- * cost time (at 4096x2049, six per step add about 16 us to a 427 us step), so a sample keeps
+ * cost time (at 4096x2049, six per step add about 16 us per step), so a sample keeps
  * the measurement from slowing the run it measures. No reference counterpart. */
 void nse2d_channel_profile_every(nse2d_channel_t ch, int every);
 /* Synchronises the stream and returns the accumulated milliseconds and launch counts per class. */
