@@ -221,17 +221,21 @@ def run_les(a):
                holdout=max(1, a.les_pairs // 20), train_steps=a.les_train_steps, batch=32, lr=1e-3,
                n_samples=min(100, a.les_pairs), log=lambda m: print("les: " + m, file=sys.stderr, flush=True))
     cl, nc, cs = r["runs"]["closure"], r["runs"]["no_closure"], r["runs"]["closure_per_step"]
+    # the value is the LES as it is meant to run: the network evaluated once per step and its term
+    # held over the three SSPRK stages (same accuracy as evaluating it in every stage, measured
+    # below, at about half the wall-clock)
     out = {
         "metric": "LES wall-clock speedup over DNS per unit simulated time (512x257 closure LES vs 4096x2049 DNS)",
-        "value": r["speedup_vs_dns"], "unit": "x", "n_gpus": 1, "higher_is_better": True, "vs_baseline": None,
+        "value": r["speedup_vs_dns_per_step_closure"], "unit": "x", "n_gpus": 1, "higher_is_better": True,
+        "vs_baseline": None,
         "dtype": "f64 solver, bf16/fp32 CNN", "data": "synthetic (DNS generated in the run)",
         "config": {"workload": "learned-closure LES 512x257 (tanh beta=2, Re_b=2e4, F0=0.05 n=12, drag 1e-3) "
                                "trained on 4096x2049 DNS filtered 8x in x (sharp spectral) and 8x in mapped y (box)",
                    "pairs": a.les_pairs, "batch": 32, "train_steps": a.les_train_steps,
-                   "spinup_time": a.les_spinup, "window_time": a.les_window},
-        "re_tau_err_closure": cl.get("re_tau_rel_err"), "re_tau_err_no_closure": nc.get("re_tau_rel_err"),
-        "speedup_closure_once_per_step": r["speedup_vs_dns_per_step_closure"],
-        "re_tau_err_closure_once_per_step": cs.get("re_tau_rel_err"),
+                   "spinup_time": a.les_spinup, "window_time": a.les_window,
+                   "closure": "evaluated once per step, held over the three stages"},
+        "re_tau_err": cs.get("re_tau_rel_err"), "re_tau_err_no_closure": nc.get("re_tau_rel_err"),
+        "speedup_closure_every_stage": r["speedup_vs_dns"], "re_tau_err_closure_every_stage": cl.get("re_tau_rel_err"),
         "U_err_closure": cl.get("U_rel_err"), "U_err_no_closure": nc.get("U_rel_err"),
         "traj_err_closure": cl.get("traj_err_end"), "traj_err_no_closure": nc.get("traj_err_end"),
         "details": r,
