@@ -336,38 +336,70 @@ __global__ void __launch_bounds__(kAllThreads) conv_lds_all_kernel(const bf16* _
     }
 }
 
-// Last layer (64 -> 1) with eight lanes per pixel: lane octet s of a pixel reads channels
-// 8s..8s+7 of each tap (a wavefront load covers 8 whole pixels, 1 KB contiguous); the octets'
-// partial sums are added with two lane exchanges.
-__global__ void __launch_bounds__(256) last_fwd_octet_kernel(const bf16* __restrict__ a, const float* __restrict__ w,
-                                                             const float* __restrict__ bias, float* __restrict__ out,
-                                                             int H, int W, int B) {
-    const int sub = threadIdx.x & 7;
-    const size_t p = (size_t)blockIdx.x * 32 + (threadIdx.x >> 3);
-    const size_t npix = (size_t)B * H * W;
-    float wr[9][8];
+// Last layer (64 -> 1): out(y, x) = b + sum over the 3x3 taps (dy, dx) and the channels c of
+// a(y+dy, x+dx, c) w[3 (dy+1) + dx+1][c]; rows outside the field contribute nothing, x wraps.
+// A workgroup takes a segment of 64 pixels of one row. For each of the three input rows and each
+// of the 66 pixels the segment reaches, eight lanes (eight channels each, one 16-byte load) form
+// the row's three partial sums s[dy][dx][px], reduced over the lanes and kept in LDS; the 64
+// outputs then add nine of them. Each input pixel is read once per output row that needs it,
+// instead of once per tap.
+constexpr int kLastThreads = 256;
+__global__ void __launch_bounds__(kLastThreads) last_fwd_rows_kernel(const bf16* __restrict__ a,
+                                                                    const float* __restrict__ w,
+                                                                    const float* __restrict__ bias,
+                                                                    float* __restrict__ out, int H, int W) {
+    __shared__ float sp[3][3][68];
+    const int nseg = W >> 6;
+    const int seg = blockIdx.x % nseg, y = (blockIdx.x / nseg) % H, b = blockIdx.x / (nseg * H);
+    const int x0 = seg << 6;
+    const int tid = threadIdx.x, sub = tid & 7, grp = tid >> 3;  // 32 octets
 #pragma unroll
-    for (int t = 0; t < 9; ++t)
+    for (int r = 0; r < 3; ++r) {
+        const int yy = y + r - 1;
+        float wr[3][8];
 #pragma unroll
-        for (int q = 0; q < 8; ++q) wr[t][q] = w[t * kC + sub * 8 + q];
-    float s = 0.0f;
-    if (p < npix) {
-        const int x = (int)(p % W), y = (int)((p / W) % H), b = (int)(p / ((size_t)W * H));
+        for (int k = 0; k < 3; ++k)
 #pragma unroll
-        for (int t = 0; t < 9; ++t) {
-            const int yy = y + t / 3 - 1;
-            if (yy < 0 || yy >= H) continue;
-            int xx = x + t % 3 - 1;
-            xx = xx < 0 ? xx + W : (xx >= W ? xx - W : xx);
-            const bf16x8 v = load8(a + (((size_t)b * H + yy) * W + xx) * kC + sub * 8);
+            for (int q = 0; q < 8; ++q) wr[k][q] = w[(r * 3 + k) * kC + sub * 8 + q];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) s += (float)v[q] * wr[t][q];
+        for (int i = 0; i < 3; ++i) {
+            const int px = grp + 32 * i;  // input pixel x0 - 1 + px
+            float s0 = 0.0f, s1 = 0.0f, s2 = 0.0f;
+            if (px < 66 && yy >= 0 && yy < H) {
+                int xx = x0 - 1 + px;
+                xx = xx < 0 ? xx + W : (xx >= W ? xx - W : xx);
+                const bf16x8 v = load8(a + (((size_t)b * H + yy) * W + xx) * kC + sub * 8);
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const float f = (float)v[q];
+                    s0 += f * wr[0][q];
+                    s1 += f * wr[1][q];
+                    s2 += f * wr[2][q];
+                }
+            }
+#pragma unroll
+            for (int m = 1; m < 8; m <<= 1) {
+                s0 += __shfl_xor(s0, m, 64);
+                s1 += __shfl_xor(s1, m, 64);
+                s2 += __shfl_xor(s2, m, 64);
+            }
+            if (sub == 0 && px < 66) {
+                sp[r][0][px] = s0;
+                sp[r][1][px] = s1;
+                sp[r][2][px] = s2;
+            }
         }
     }
-    s += __shfl_xor(s, 1, 64);
-    s += __shfl_xor(s, 2, 64);
-    s += __shfl_xor(s, 4, 64);
-    if (sub == 0 && p < npix) out[p] = s + bias[0];
+    __syncthreads();
+    if (tid < 64) {
+        // output pixel x0 + tid takes tap (dy, dx) from input pixel x0 + tid + dx, i.e. px = tid + dx + 1
+        float o = 0.0f;
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) o += sp[r][k][tid + k];
+        out[((size_t)b * H + y) * W + x0 + tid] = o + bias[0];
+    }
 }
 
 // Partial weight gradient of a 64-output layer over a strip of row segments:
@@ -543,12 +575,28 @@ __global__ void __launch_bounds__(256) last_wgrad_kernel(const float* __restrict
 }
 
 // Inputs: fp32 [b][y][x][4] (already normalised) -> bf16 NHWC padded to 32 channels.
+// One thread per pixel: its four inputs (one 16-byte load) go to channels 0..3 of its 32 bf16
+// channels, zeros to the rest (four 16-byte stores).
 __global__ void pack_input_kernel(const float* __restrict__ in, bf16* __restrict__ out, size_t npix) {
-    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= npix * kCin0) return;
-    const size_t p = e / kCin0;
-    const int c = (int)(e % kCin0);
-    out[e] = (bf16)(c < kInputs ? in[p * kInputs + c] : 0.0f);
+    static_assert(kInputs == 4 && kCin0 == 32, "one float4 in, four bf16x8 out");
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= npix) return;
+    const float4 v = *reinterpret_cast<const float4*>(in + p * kInputs);
+    bf16x8 first;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) first[k] = (bf16)0.0f;
+    first[0] = (bf16)v.x;
+    first[1] = (bf16)v.y;
+    first[2] = (bf16)v.z;
+    first[3] = (bf16)v.w;
+    bf16x8 zero;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) zero[k] = (bf16)0.0f;
+    bf16x8* o = reinterpret_cast<bf16x8*>(out + p * kCin0);
+    o[0] = first;
+    o[1] = zero;
+    o[2] = zero;
+    o[3] = zero;
 }
 
 // Mean squared error over the rows 1..H-2 (the wall rows carry no subfilter flux) and its gradient.
@@ -660,17 +708,18 @@ void launch_conv(CnnImpl* c, const cnn::bf16* in, const cnn::bf16* w, const floa
 }
 
 // keep = false (inference): the pre-activations z are not stored, so no backward may follow.
-void forward(CnnImpl* c, const float* d_in, int B, bool keep = true) {
+// out: where the network's output goes (nullptr: the model's own buffer, which the loss reads)
+void forward(CnnImpl* c, const float* d_in, int B, bool keep = true, float* out = nullptr) {
     const size_t npix = (size_t)B * c->H * c->W;
-    cnn::pack_input_kernel<<<ceil_div(npix * cnn::kCin0, 256), 256, 0, c->s>>>(d_in, c->x0, npix);
+    cnn::pack_input_kernel<<<ceil_div(npix, 256), 256, 0, c->s>>>(d_in, c->x0, npix);
     NSE2D_LAUNCH_CHECK();
     launch_conv<cnn::kCin0, cnn::MODE_FWD_GELU>(c, c->x0, c->wb[0], c->p + c->off_b[0], keep ? c->z[0] : nullptr,
                                                 c->a[0], nullptr, B);
     for (int l = 1; l < 5; ++l)
         launch_conv<cnn::kC, cnn::MODE_FWD_GELU>(c, c->a[l - 1], c->wb[l], c->p + c->off_b[l],
                                                  keep ? c->z[l] : nullptr, c->a[l], nullptr, B);
-    cnn::last_fwd_octet_kernel<<<ceil_div(npix, 32), 256, 0, c->s>>>(c->a[4], c->p + c->off_w[5],
-                                                                    c->p + c->off_b[5], c->out, c->H, c->W, B);
+    cnn::last_fwd_rows_kernel<<<B * c->H * (c->W / 64), cnn::kLastThreads, 0, c->s>>>(
+        c->a[4], c->p + c->off_w[5], c->p + c->off_b[5], out ? out : c->out, c->H, c->W);
     NSE2D_LAUNCH_CHECK();
 }
 
@@ -889,10 +938,7 @@ int nse2d_cnn_forward(nse2d_cnn_t c, const float* d_in, float* d_out, int B) {
 int nse2d_cnn_infer(nse2d_cnn_t c, const float* d_in, float* d_out, int B) {
     CnnImpl* k = c->impl;
     if (B < 1 || B > k->maxB) { set_last_error("nse2d_cnn_infer: batch outside [1, max_batch]"); return -1; }
-    forward(k, d_in, B, false);
-    if (d_out)
-        NSE2D_HIP_CHECK(hipMemcpyAsync(d_out, k->out, sizeof(float) * (size_t)B * k->H * k->W, hipMemcpyDeviceToDevice,
-                                       k->s));
+    forward(k, d_in, B, false, d_out);  // the last layer writes d_out directly
     return 0;
 }
 int nse2d_cnn_set_direct(nse2d_cnn_t c, int on) {
