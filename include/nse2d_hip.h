@@ -332,7 +332,9 @@ size_t nse2d_cnn_num_params(nse2d_cnn_t c);
 void nse2d_cnn_get_params(nse2d_cnn_t c, float* host);
 void nse2d_cnn_set_params(nse2d_cnn_t c, const float* host);
 void nse2d_cnn_get_grads(nse2d_cnn_t c, float* host);
-/* d_in: [B][H][W][4] fp32 (normalised inputs); d_out: [B][H][W] fp32 (may be NULL). 0 on success. */
+/* d_in: [B][H][W][4] fp32 (normalised inputs), 16-byte aligned; d_out: [B][H][W] fp32 (may be
+ * NULL). 0 on success; -1 (nse2d_last_error) for a batch outside [1, max_batch] or a misaligned
+ * d_in, in this and the other entry points that take d_in. */
 int nse2d_cnn_forward(nse2d_cnn_t c, const float* d_in, float* d_out, int B);
 /* The forward of nse2d_cnn_forward without keeping the pre-activations (inference: no backward
  * may follow). 0 on success. */

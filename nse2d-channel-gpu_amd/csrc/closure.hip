@@ -22,6 +22,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
 #include <cstdlib>
 #include <cstring>
 #include <string>
@@ -132,7 +133,7 @@ __global__ void __launch_bounds__(256) conv_lds_kernel(const bf16* __restrict__ 
                                                        bf16* __restrict__ aout, const bf16* __restrict__ zprev,
                                                        int H, int W) {
     using S = ConvLds<CIN>;
-    constexpr int CP = S::CP, C8 = CIN / 8;
+    constexpr int C8 = CIN / 8;
     __shared__ alignas(16) S sh;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nseg = W >> 6, nyt = (H + kTileRows - 1) / kTileRows;
@@ -479,99 +480,217 @@ __global__ void __launch_bounds__(CIN * 4) conv_wgrad_kernel(const bf16* __restr
         }
 }
 
-// Ordered sum of nb partial blocks of n floats: out[j] = sum_k partial[k][j].
-__global__ void sum_partials_kernel(const float* __restrict__ partial, int nb, int n, float* __restrict__ out) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
+// Ordered sum of nb partial blocks of n floats: out[j] = sum_k partial[k][j]. A workgroup of 256
+// threads takes cols consecutive columns (cols divides 256); its 256 / cols groups of threads add
+// interleaved blocks k = g, g + groups, ... and the group sums are added in a fixed order, so the
+// result does not depend on timing.
+constexpr int kSumThreads = 256;
+constexpr int kBiasPixPerBlock = 4096;  // pixels per bias-gradient workgroup
+constexpr int kLastWgradBlocks = 1024;  // most workgroups of the last layer's weight gradient
+__global__ void __launch_bounds__(kSumThreads) sum_partials_kernel(const float* __restrict__ partial, int nb, int n,
+                                                                   int cols, float* __restrict__ out) {
+    __shared__ float sm[kSumThreads];
+    const int groups = kSumThreads / cols;
+    const int c = threadIdx.x % cols, g = threadIdx.x / cols;
+    const int j = blockIdx.x * cols + c;
     float s = 0.0f;
-    for (int k = 0; k < nb; ++k) s += partial[(size_t)k * n + j];
-    out[j] = s;
+    if (j < n) {
+        int k = g;
+        float s1 = 0.0f, s2 = 0.0f, s3 = 0.0f;
+        for (; k + 3 * groups < nb; k += 4 * groups) {
+            s += partial[(size_t)k * n + j];
+            s1 += partial[(size_t)(k + groups) * n + j];
+            s2 += partial[(size_t)(k + 2 * groups) * n + j];
+            s3 += partial[(size_t)(k + 3 * groups) * n + j];
+        }
+        for (; k < nb; k += groups) s += partial[(size_t)k * n + j];
+        s = (s + s1) + (s2 + s3);
+    }
+    sm[threadIdx.x] = s;
+    __syncthreads();
+    if (g == 0 && j < n) {
+        float t = 0.0f;
+        for (int q = 0; q < groups; ++q) t += sm[q * cols + c];
+        out[j] = t;
+    }
 }
 
-// Bias gradient of a 64-channel layer: per-block partial column sums over pixels.
+// Sum over the eight octets of a wavefront (lanes l, l^8, l^16, ... hold the same channels).
+__device__ inline float octet_sum(float v) {
+    v += __shfl_xor(v, 8, 64);
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
+    return v;
+}
+
+// Bias gradient of a 64-channel layer: per-block partial column sums over a run of pixels. Eight
+// lanes cover a pixel's 64 channels with one 16-byte load each; a workgroup reads 32 pixels a pass.
 __global__ void __launch_bounds__(256) bias_grad_kernel(const bf16* __restrict__ dz, size_t npix, int pix_per_block,
                                                         float* __restrict__ partial) {
     __shared__ float sm[4][kC];
-    const int c = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const int tid = threadIdx.x, sub = tid & 7, grp = tid >> 3, wave = tid >> 6;
     const size_t p0 = (size_t)blockIdx.x * pix_per_block;
     const size_t p1 = p0 + pix_per_block < npix ? p0 + pix_per_block : npix;
-    float s = 0.0f;
-    for (size_t p = p0 + g; p < p1; p += 4) s += (float)dz[p * kC + c];
-    sm[g][c] = s;
+    float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    size_t p = p0 + grp;
+    for (; p + 32 < p1; p += 64) {
+        const bf16x8 v0 = load8(dz + p * kC + sub * 8);
+        const bf16x8 v1 = load8(dz + (p + 32) * kC + sub * 8);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) acc[q] += (float)v0[q] + (float)v1[q];
+    }
+    if (p < p1) {
+        const bf16x8 v0 = load8(dz + p * kC + sub * 8);
+#pragma unroll
+        for (int q = 0; q < 8; ++q) acc[q] += (float)v0[q];
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) acc[q] = octet_sum(acc[q]);
+    if ((tid & 63) < 8)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) sm[wave][sub * 8 + q] = acc[q];
     __syncthreads();
-    if (g == 0) partial[(size_t)blockIdx.x * kC + c] = sm[0][c] + sm[1][c] + sm[2][c] + sm[3][c];
+    if (tid < kC) partial[(size_t)blockIdx.x * kC + tid] = (sm[0][tid] + sm[1][tid]) + (sm[2][tid] + sm[3][tid]);
 }
 
-// Last layer (64 -> 1, linear): out[p] = b + sum_t sum_ci A[p + off_t][ci] w[t][ci], fp32.
-__global__ void last_fwd_kernel(const bf16* __restrict__ a, const float* __restrict__ w, const float* __restrict__ bias,
-                                float* __restrict__ out, int H, int W, int B) {
-    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= (size_t)B * H * W) return;
-    const int x = (int)(p % W), y = (int)((p / W) % H), b = (int)(p / ((size_t)W * H));
-    float s = bias[0];
-    for (int t = 0; t < 9; ++t) {
-        const int yy = y + t / 3 - 1;
-        if (yy < 0 || yy >= H) continue;
-        int xx = x + t % 3 - 1;
-        xx = xx < 0 ? xx + W : (xx >= W ? xx - W : xx);
-        const bf16* row = a + (((size_t)b * H + yy) * W + xx) * kC;
-        for (int c8 = 0; c8 < kC; c8 += 8) {
-            const bf16x8 v = load8(row + c8);
-#pragma unroll
-            for (int q = 0; q < 8; ++q) s += (float)v[q] * w[t * kC + c8 + q];
-        }
-    }
-    out[p] = s;
+// The output gradient g of the last layer around a row segment: gs[dyi][j] = g(y + 1 - dyi, x0 - 1 + j)
+// for j = 0..65 (x wraps; rows outside the field give 0). Tap (dy, dx) = (dyi - 1, dxi - 1) of the
+// pixel x0 + px takes g(y - dy, x0 + px - dx) = gs[dyi][px + 2 - dxi]. Threads 0..197 load one value.
+__device__ inline float load_g_ring(const float* __restrict__ g, int b, int y, int x0, int H, int W, int tid) {
+    if (tid >= 198) return 0.0f;
+    const int dyi = tid / 66, j = tid % 66;
+    const int yy = y + 1 - dyi;
+    if (yy < 0 || yy >= H) return 0.0f;
+    int xx = x0 - 1 + j;
+    xx = xx < 0 ? xx + W : (xx >= W ? xx - W : xx);
+    return g[((size_t)b * H + yy) * W + xx];
 }
 
 // Input gradient of the last layer, through the GELU of the layer before:
 //   dz5[p][ci] = gelu'(z5[p][ci]) * sum_t g[p - off_t] w[t][ci]
-__global__ void last_bwd_data_kernel(const float* __restrict__ g, const float* __restrict__ w,
-                                     const bf16* __restrict__ z5, bf16* __restrict__ dz5, int H, int W, int B) {
-    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (size_t)B * H * W * kC) return;
-    const int ci = (int)(e % kC);
-    const size_t p = e / kC;
-    const int x = (int)(p % W), y = (int)((p / W) % H), b = (int)(p / ((size_t)W * H));
-    float s = 0.0f;
-    for (int t = 0; t < 9; ++t) {
-        const int yy = y - (t / 3 - 1);
-        if (yy < 0 || yy >= H) continue;
-        int xx = x - (t % 3 - 1);
-        xx = xx < 0 ? xx + W : (xx >= W ? xx - W : xx);
-        s += g[((size_t)b * H + yy) * W + xx] * w[t * kC + ci];
+// One workgroup per row segment of 64 pixels at a time (grid-stride over the segments): the
+// segment's g ring goes to LDS, eight lanes take a pixel's 64 channels (16-byte loads and stores),
+// each octet two pixels.
+__global__ void __launch_bounds__(256) last_bwd_data_kernel(const float* __restrict__ g, const float* __restrict__ w,
+                                                            const bf16* __restrict__ z5, bf16* __restrict__ dz5,
+                                                            int H, int W, int B) {
+    __shared__ float gs[198];
+    const int tid = threadIdx.x, sub = tid & 7, grp = tid >> 3;
+    float wr[9][8];
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) wr[t][q] = w[t * kC + sub * 8 + q];
+    const int nseg = W >> 6, total = B * H * nseg;
+    // the loads of segment sg + gridDim.x are issued before segment sg is computed (clamped to the
+    // last segment, so they need no branch)
+    auto seg_row = [&](int sg, int& b, int& y, int& x0) {
+        x0 = (sg % nseg) << 6;
+        y = (sg / nseg) % H;
+        b = sg / (nseg * H);
+    };
+    int b, y, x0;
+    seg_row(blockIdx.x, b, y, x0);
+    float gv = load_g_ring(g, b, y, x0, H, W, tid);
+    bf16x8 z0 = load8(z5 + ((((size_t)b * H + y) * W + x0) + grp) * kC + sub * 8);
+    bf16x8 z1 = load8(z5 + ((((size_t)b * H + y) * W + x0) + grp + 32) * kC + sub * 8);
+    for (int sg = blockIdx.x; sg < total; sg += gridDim.x) {
+        const size_t row = ((size_t)b * H + y) * W + x0;
+        if (tid < 198) gs[tid] = gv;
+        const bf16x8 zc0 = z0, zc1 = z1;
+        seg_row(min(sg + (int)gridDim.x, total - 1), b, y, x0);
+        gv = load_g_ring(g, b, y, x0, H, W, tid);
+        z0 = load8(z5 + ((((size_t)b * H + y) * W + x0) + grp) * kC + sub * 8);
+        z1 = load8(z5 + ((((size_t)b * H + y) * W + x0) + grp + 32) * kC + sub * 8);
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int px = grp + 32 * h;
+            float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+            for (int t = 0; t < 9; ++t) {
+                const float gt = gs[(t / 3) * 66 + px + 2 - (t % 3)];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) s[q] += gt * wr[t][q];
+            }
+            const bf16x8 z = h ? zc1 : zc0;
+            bf16x8 o;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) o[q] = (bf16)(s[q] * gelu_grad((float)z[q]));
+            *reinterpret_cast<bf16x8*>(dz5 + (row + px) * kC + sub * 8) = o;
+        }
+        __syncthreads();
     }
-    dz5[e] = (bf16)(s * gelu_grad((float)z5[e]));
 }
 
-// Weight and bias gradient partials of the last layer: one block per pixel chunk, 9*64 + 1 sums.
+// Weight and bias gradient partials of the last layer over a run of row segments:
+//   dW[t][ci] = sum_q a[q][ci] g[q - off_t],  db = sum_p g[p]
+// taken over input pixels q, each read once (one 16-byte load per lane, eight lanes a pixel);
+// the segment's g ring sits in LDS. Output per block: 9 * 64 weight sums, then the bias sum.
 __global__ void __launch_bounds__(256) last_wgrad_kernel(const float* __restrict__ g, const bf16* __restrict__ a,
-                                                         int H, int W, int B, int pix_per_block,
+                                                         int H, int W, int B, int segs_per_block,
                                                          float* __restrict__ partial) {
+    __shared__ float gs[198];
     __shared__ float sm[4][9 * kC + 1];
-    const int tid = threadIdx.x, c = tid & 63, grp = tid >> 6;
-    const size_t np = (size_t)B * H * W;
-    const size_t p0 = (size_t)blockIdx.x * pix_per_block, p1 = p0 + pix_per_block < np ? p0 + pix_per_block : np;
-    float acc[9];
-    for (int t = 0; t < 9; ++t) acc[t] = 0.0f;
-    float gb = 0.0f;
-    for (size_t p = p0 + grp; p < p1; p += 4) {
-        const int x = (int)(p % W), y = (int)((p / W) % H), b = (int)(p / ((size_t)W * H));
-        const float gv = g[p];
-        if (c == 0) gb += gv;
-        for (int t = 0; t < 9; ++t) {
-            const int yy = y + t / 3 - 1;
-            if (yy < 0 || yy >= H) continue;
-            int xx = x + t % 3 - 1;
-            xx = xx < 0 ? xx + W : (xx >= W ? xx - W : xx);
-            acc[t] += gv * (float)a[(((size_t)b * H + yy) * W + xx) * kC + c];
+    const int tid = threadIdx.x, sub = tid & 7, grp = tid >> 3, wave = tid >> 6;
+    float acc[9][8];
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) acc[t][q] = 0.0f;
+    float gb = 0.0f;  // bias: threads 67..130 hold the segment's own row of g (dyi = 1, j = 1..64)
+    const int nseg = W >> 6, total = B * H * nseg;
+    const int s0 = blockIdx.x * segs_per_block, s1 = min(total, s0 + segs_per_block);
+    // the loads of segment sg + 1 are issued before segment sg is computed (clamped to the block's
+    // last segment, so they need no branch)
+    auto seg_row = [&](int sg, int& b, int& y, int& x0) {
+        x0 = (sg % nseg) << 6;
+        y = (sg / nseg) % H;
+        b = sg / (nseg * H);
+    };
+    int b, y, x0;
+    seg_row(s0, b, y, x0);
+    float gv = load_g_ring(g, b, y, x0, H, W, tid);
+    bf16x8 an0 = load8(a + ((((size_t)b * H + y) * W + x0) + grp) * kC + sub * 8);
+    bf16x8 an1 = load8(a + ((((size_t)b * H + y) * W + x0) + grp + 32) * kC + sub * 8);
+    for (int sg = s0; sg < s1; ++sg) {
+        if (tid < 198) gs[tid] = gv;
+        if (tid >= 67 && tid <= 130) gb += gv;
+        const bf16x8 a0 = an0, a1 = an1;
+        seg_row(min(sg + 1, s1 - 1), b, y, x0);
+        gv = load_g_ring(g, b, y, x0, H, W, tid);
+        an0 = load8(a + ((((size_t)b * H + y) * W + x0) + grp) * kC + sub * 8);
+        an1 = load8(a + ((((size_t)b * H + y) * W + x0) + grp + 32) * kC + sub * 8);
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int px = grp + 32 * h;
+            const bf16x8 v = h ? a1 : a0;
+#pragma unroll
+            for (int t = 0; t < 9; ++t) {
+                const float gt = gs[(t / 3) * 66 + px + 2 - (t % 3)];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) acc[t][q] += (float)v[q] * gt;
+            }
         }
+        __syncthreads();
     }
-    for (int t = 0; t < 9; ++t) sm[grp][t * kC + c] = acc[t];
-    if (c == 0) sm[grp][9 * kC] = gb;
+#pragma unroll
+    for (int t = 0; t < 9; ++t)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) acc[t][q] = octet_sum(acc[t][q]);
+    if ((tid & 63) < 8)
+#pragma unroll
+        for (int t = 0; t < 9; ++t)
+#pragma unroll
+            for (int q = 0; q < 8; ++q) sm[wave][t * kC + sub * 8 + q] = acc[t][q];
+    // bias: wavefront sums, then the four wavefronts
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) gb += __shfl_xor(gb, m, 64);
+    if ((tid & 63) == 0) sm[wave][9 * kC] = gb;
     __syncthreads();
     for (int j = tid; j < 9 * kC + 1; j += 256)
-        partial[(size_t)blockIdx.x * (9 * kC + 1) + j] = sm[0][j] + sm[1][j] + sm[2][j] + sm[3][j];
+        partial[(size_t)blockIdx.x * (9 * kC + 1) + j] = (sm[0][j] + sm[1][j]) + (sm[2][j] + sm[3][j]);
 }
 
 // Inputs: fp32 [b][y][x][4] (already normalised) -> bf16 NHWC padded to 32 channels.
@@ -723,6 +842,16 @@ void forward(CnnImpl* c, const float* d_in, int B, bool keep = true, float* out 
     NSE2D_LAUNCH_CHECK();
 }
 
+// out[j] = sum over nb blocks of partial[k][j], j < n
+void sum_partials(CnnImpl* c, int nb, int n, float* out) {
+    const int cols = n >= 16384 ? 64 : 16;
+    cnn::sum_partials_kernel<<<ceil_div(n, cols), cnn::kSumThreads, 0, c->s>>>(c->partial, nb, n, cols, out);
+    NSE2D_LAUNCH_CHECK();
+}
+
+// Row segments of 64 pixels covering a batch
+int row_segments(const CnnImpl* c, int B) { return B * c->H * (c->W / 64); }
+
 template <int CIN>
 void wgrad(CnnImpl* c, const cnn::bf16* x, const cnn::bf16* dz, float* gw, float* gb, int B) {
     const int total = B * c->H * (c->W / 64);
@@ -731,30 +860,26 @@ void wgrad(CnnImpl* c, const cnn::bf16* x, const cnn::bf16* dz, float* gw, float
     const int used = (total + per - 1) / per;
     cnn::conv_wgrad_kernel<CIN><<<used, CIN * 4, 0, c->s>>>(x, dz, c->partial, c->H, c->W, B, per);
     NSE2D_LAUNCH_CHECK();
-    const int n = 9 * cnn::kC * CIN;
-    cnn::sum_partials_kernel<<<ceil_div(n, 256), 256, 0, c->s>>>(c->partial, used, n, gw);
-    NSE2D_LAUNCH_CHECK();
+    sum_partials(c, used, 9 * cnn::kC * CIN, gw);
     const size_t npix = (size_t)B * c->H * c->W;
-    const int ppb = 4096;
-    const int nbb = (int)ceil_div(npix, ppb);
-    cnn::bias_grad_kernel<<<nbb, 256, 0, c->s>>>(dz, npix, ppb, c->partial);
+    const int nbb = (int)ceil_div(npix, cnn::kBiasPixPerBlock);
+    cnn::bias_grad_kernel<<<nbb, 256, 0, c->s>>>(dz, npix, cnn::kBiasPixPerBlock, c->partial);
     NSE2D_LAUNCH_CHECK();
-    cnn::sum_partials_kernel<<<1, 64, 0, c->s>>>(c->partial, nbb, cnn::kC, gb);
-    NSE2D_LAUNCH_CHECK();
+    sum_partials(c, nbb, cnn::kC, gb);
 }
 
 // Gradients of sum(gout * out) with respect to every parameter, after forward() on the same batch.
 void backward(CnnImpl* c, const float* gout, int B) {
-    const size_t npix = (size_t)B * c->H * c->W;
     {   // last layer
-        const int ppb = 2048, nb = (int)ceil_div(npix, ppb);
-        cnn::last_wgrad_kernel<<<nb, 256, 0, c->s>>>(gout, c->a[4], c->H, c->W, B, ppb, c->partial);
+        const int total = row_segments(c, B);
+        const int per = ceil_div(total, std::min(total, cnn::kLastWgradBlocks));
+        const int used = ceil_div(total, per);
+        cnn::last_wgrad_kernel<<<used, 256, 0, c->s>>>(gout, c->a[4], c->H, c->W, B, per, c->partial);
         NSE2D_LAUNCH_CHECK();
-        cnn::sum_partials_kernel<<<ceil_div(9 * cnn::kC + 1, 256), 256, 0, c->s>>>(c->partial, nb, 9 * cnn::kC + 1,
-                                                                                   c->g + c->off_w[5]);
-        NSE2D_LAUNCH_CHECK();
-        cnn::last_bwd_data_kernel<<<ceil_div(npix * cnn::kC, 256), 256, 0, c->s>>>(gout, c->p + c->off_w[5], c->z[4],
-                                                                                  c->dz0, c->H, c->W, B);
+        sum_partials(c, used, 9 * cnn::kC + 1, c->g + c->off_w[5]);
+        // four workgroups per CU are resident at the kernel's register count; each loops over segments
+        cnn::last_bwd_data_kernel<<<std::min(total, 4 * c->cus), 256, 0, c->s>>>(gout, c->p + c->off_w[5], c->z[4],
+                                                                                 c->dz0, c->H, c->W, B);
         NSE2D_LAUNCH_CHECK();
     }
     cnn::bf16* cur = c->dz0;
@@ -857,8 +982,9 @@ CnnImpl* cnn_create(int H, int W, int maxB, unsigned long long seed, hipStream_t
     NSE2D_HIP_CHECK(hipMalloc(&c->gout, sizeof(float) * npix));
     c->wgrad_blocks = 1024;
     const size_t pw = (size_t)c->wgrad_blocks * 9 * cnn::kC * cnn::kC;
-    const size_t pb = ceil_div(npix, 4096) * (size_t)cnn::kC + ceil_div(npix, 2048) * (size_t)(9 * cnn::kC + 1);
-    c->partial_floats = std::max(pw, pb);
+    const size_t pb = ceil_div(npix, cnn::kBiasPixPerBlock) * (size_t)cnn::kC;
+    const size_t pl = (size_t)std::min(row_segments(c, maxB), cnn::kLastWgradBlocks) * (9 * cnn::kC + 1);
+    c->partial_floats = std::max(pw, std::max(pb, pl));
     NSE2D_HIP_CHECK(hipMalloc(&c->partial, sizeof(float) * c->partial_floats));
     NSE2D_HIP_CHECK(hipMalloc(&c->loss_part, sizeof(float) * 1024));
     refresh_bf16(c);
@@ -926,9 +1052,17 @@ void nse2d_cnn_get_grads(nse2d_cnn_t c, float* host) {
     NSE2D_HIP_CHECK(hipMemcpyAsync(host, c->impl->g, sizeof(float) * c->impl->nparam, hipMemcpyDeviceToHost, c->impl->s));
     NSE2D_HIP_CHECK(hipStreamSynchronize(c->impl->s));
 }
+// The input packing reads a pixel's four inputs as one 16-byte load.
+static bool input_misaligned(const float* d_in, const char* who) {
+    if (reinterpret_cast<uintptr_t>(d_in) % 16 == 0) return false;
+    set_last_error(std::string(who) + ": d_in must be 16-byte aligned");
+    return true;
+}
+
 int nse2d_cnn_forward(nse2d_cnn_t c, const float* d_in, float* d_out, int B) {
     CnnImpl* k = c->impl;
     if (B < 1 || B > k->maxB) { set_last_error("nse2d_cnn_forward: batch outside [1, max_batch]"); return -1; }
+    if (input_misaligned(d_in, "nse2d_cnn_forward")) return -1;
     forward(k, d_in, B);
     if (d_out)
         NSE2D_HIP_CHECK(hipMemcpyAsync(d_out, k->out, sizeof(float) * (size_t)B * k->H * k->W, hipMemcpyDeviceToDevice,
@@ -938,6 +1072,7 @@ int nse2d_cnn_forward(nse2d_cnn_t c, const float* d_in, float* d_out, int B) {
 int nse2d_cnn_infer(nse2d_cnn_t c, const float* d_in, float* d_out, int B) {
     CnnImpl* k = c->impl;
     if (B < 1 || B > k->maxB) { set_last_error("nse2d_cnn_infer: batch outside [1, max_batch]"); return -1; }
+    if (input_misaligned(d_in, "nse2d_cnn_infer")) return -1;
     forward(k, d_in, B, false, d_out);  // the last layer writes d_out directly
     return 0;
 }
@@ -955,6 +1090,7 @@ int nse2d_cnn_backward(nse2d_cnn_t c, const float* d_gout, int B) {
 double nse2d_cnn_train_step(nse2d_cnn_t c, const float* d_in, const float* d_target, int B, float lr) {
     CnnImpl* k = c->impl;
     if (B < 1 || B > k->maxB) { set_last_error("nse2d_cnn_train_step: batch outside [1, max_batch]"); return -1.0; }
+    if (input_misaligned(d_in, "nse2d_cnn_train_step")) return -1.0;
     forward(k, d_in, B);
     const double loss = cnn_loss_and_grad(k, d_target, B);
     backward(k, k->gout, B);
@@ -964,6 +1100,7 @@ double nse2d_cnn_train_step(nse2d_cnn_t c, const float* d_in, const float* d_tar
 double nse2d_cnn_eval_loss(nse2d_cnn_t c, const float* d_in, const float* d_target, int B) {
     CnnImpl* k = c->impl;
     if (B < 1 || B > k->maxB) { set_last_error("nse2d_cnn_eval_loss: batch outside [1, max_batch]"); return -1.0; }
+    if (input_misaligned(d_in, "nse2d_cnn_eval_loss")) return -1.0;
     forward(k, d_in, B);
     return cnn_loss_and_grad(k, d_target, B);
 }

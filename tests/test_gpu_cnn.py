@@ -31,7 +31,9 @@ def make(H, W, B, seed=5):
     return c, p
 
 
-@pytest.mark.parametrize("H,W,B", [(9, 64, 2), (7, 128, 1)])
+# 33x512x4 has 1056 row segments of 64 pixels: more than the last layer's gradient kernels take
+# in one pass (1024 workgroups), so their segment loops run more than once
+@pytest.mark.parametrize("H,W,B", [(9, 64, 2), (7, 128, 1), (33, 512, 4)])
 def test_forward_and_gradients_match_oracle(H, W, B):
     print(f"[cnn] {B}x{H}x{W}", flush=True)
     c, p = make(H, W, B)
@@ -100,6 +102,38 @@ def test_training_reduces_loss_on_a_fixed_batch():
     assert abs(losses[0] - co.mse(out, tgt)[0]) <= 1e-2 * losses[0]
     assert losses[-1] < 0.2 * losses[0]
     nh.lib().nse2d_cnn_destroy(c)
+
+
+def test_last_layer_gradient_identities_at_full_size():
+    """The last layer is linear in its weights and bias, so for any output gradient g:
+    sum_p g[p] (out[p] - b) = <dW, w> and db = sum_p g[p]. With g = out - b the left side is a sum
+    of squares (no cancellation), and a weight gradient that gathers any tap, edge column or edge
+    row from other pixels than the forward does leaves the identity by far more than the fp32
+    rounding of the two sums. At 257x512x2 (4112 row segments, several per workgroup)."""
+    H, W, B = 257, 512, 2
+    print(f"[cnn] last-layer identities {B}x{H}x{W}", flush=True)
+    c, p = make(H, W, B, seed=3)
+    lay, n = co.layout()
+    ow, cout, cin, ob = lay[-1]
+    rng = np.random.default_rng(17)
+    inp = rng.normal(size=(B, H, W, co.NIN)).astype(np.float32)
+    d_in, d_out = nh.DeviceArray(inp), nh.DeviceArray(shape=(B, H, W))
+    assert nh.lib().nse2d_cnn_forward(c, d_in.ptr, d_out.ptr, B) == 0
+    out = d_out.download()
+    gout = (out - p[ob]).astype(np.float32)
+    d_g = nh.DeviceArray(gout)
+    assert nh.lib().nse2d_cnn_backward(c, d_g.ptr, B) == 0
+    g = np.zeros(n, np.float32)
+    nh.lib().nse2d_cnn_get_grads(c, g.ctypes.data)
+    nh.lib().nse2d_cnn_destroy(c)
+    w = p[ow:ow + 9 * cout * cin].astype(np.float64)
+    dw = g[ow:ow + 9 * cout * cin].astype(np.float64)
+    g64 = gout.astype(np.float64)
+    lhs = float(np.sum(g64 * (out.astype(np.float64) - float(p[ob]))))
+    rhs = float(np.dot(dw, w))
+    print(f"[cnn] <g, out - b> {lhs:.8e}, <dW, w> {rhs:.8e}, rel {abs(lhs - rhs) / lhs:.2e}", flush=True)
+    assert abs(lhs - rhs) <= 1e-6 * lhs
+    assert abs(float(g[ob]) - float(np.sum(g64))) <= 1e-5 * float(np.sum(np.abs(g64)))
 
 
 @pytest.mark.parametrize("H,W,B", [(9, 64, 2), (7, 128, 1), (33, 64, 3), (257, 512, 2)])
