@@ -403,20 +403,48 @@ __global__ void __launch_bounds__(kLastThreads) last_fwd_rows_kernel(const bf16*
     }
 }
 
+// Two bf16 values as one 32-bit word (lo in the low half).
+__device__ inline unsigned pack_bf16x2(bf16 lo, bf16 hi) {
+    return (unsigned)__builtin_bit_cast(unsigned short, lo) | ((unsigned)__builtin_bit_cast(unsigned short, hi) << 16);
+}
+
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+// Elements s .. s+7 of the 16 bf16 values c0 (0..7), c1 (8..15), for s = 0, 1, 2.
+template <int S>
+__device__ inline bf16x8 window8(const bf16x8& c0, const bf16x8& c1) {
+    if constexpr (S == 0) return c0;
+    const u32x4 d = __builtin_bit_cast(u32x4, c0), e = __builtin_bit_cast(u32x4, c1);
+    u32x4 r;
+    if constexpr (S == 1) {
+        r[0] = __builtin_amdgcn_alignbit(d[1], d[0], 16);
+        r[1] = __builtin_amdgcn_alignbit(d[2], d[1], 16);
+        r[2] = __builtin_amdgcn_alignbit(d[3], d[2], 16);
+        r[3] = __builtin_amdgcn_alignbit(e[0], d[3], 16);
+    } else {
+        r[0] = d[1];
+        r[1] = d[2];
+        r[2] = d[3];
+        r[3] = e[0];
+    }
+    return __builtin_bit_cast(bf16x8, r);
+}
+
 // Partial weight gradient of a 64-output layer over a strip of row segments:
 //   dW[t][co][ci] += sum_pixels dZ[p][co] * X[p + off_t][ci]
 // Workgroup: (CIN/32)*2 wavefronts, wavefront w owns co block w/(CIN/32), ci block w%(CIN/32), all 9
-// taps (9 accumulators of 16). LDS: dZ^T [64][64 px] and the input rows y-1..y+1 transposed and
-// shifted by dx = -1, 0, 1: X^T[dx][dy][ci][64 px].
+// taps (9 accumulators of 16). LDS holds dZ^T [64 ch][64 px] and the input rows y-1..y+1
+// transposed, X^T[dy][ci][j] for the pixels x0 - 1 + j, j = 0..65. Both are staged a pixel pair at
+// a time (two 16-byte loads, eight 4-byte LDS writes). The operand of tap dx starts at j = p + dx + 1:
+// it is cut from two aligned 16-byte LDS reads by funnel shifts, one pair of reads for the three dx.
 template <int CIN>
 __global__ void __launch_bounds__(CIN * 4) conv_wgrad_kernel(const bf16* __restrict__ x, const bf16* __restrict__ dz,
                                                              float* __restrict__ partial, int H, int W, int B,
                                                              int segs_per_block) {
     constexpr int NT = CIN * 4;  // threads
-    constexpr int PAD = 8;       // pixel padding of the LDS rows (bank spread)
-    constexpr int LP = 64 + PAD;
+    constexpr int LP = 72;       // row length: 66 pixels used, rows 16-byte aligned
     __shared__ alignas(16) bf16 dzT[kC][LP];
-    __shared__ alignas(16) bf16 xT[3][3][CIN][LP];
+    __shared__ alignas(16) bf16 xT[3][CIN][LP];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     constexpr int NCI = CIN / 32;
     const int cb = wave / NCI, ib = wave % NCI;
@@ -429,44 +457,48 @@ __global__ void __launch_bounds__(CIN * 4) conv_wgrad_kernel(const bf16* __restr
     for (int sg = s0; sg < s1; ++sg) {
         const int seg = sg % nseg, y = (sg / nseg) % H, b = sg / (nseg * H);
         const int x0 = seg << 6;
-        // stage dZ^T: 64 px x 64 ch, 8 channels per load
-        for (int e = tid; e < 64 * (kC / 8); e += NT) {
-            const int p = e / (kC / 8), c8 = (e % (kC / 8)) * 8;
-            const bf16x8 v = load8(dz + (((size_t)b * H + y) * W + x0 + p) * kC + c8);
+        // stage dZ^T: pixel pairs (2m, 2m+1) x 8 channels
+        for (int e = tid; e < 32 * (kC / 8); e += NT) {
+            const int m = e / (kC / 8), c8 = (e % (kC / 8)) * 8;
+            const bf16* src = dz + (((size_t)b * H + y) * W + x0 + 2 * m) * kC + c8;
+            const bf16x8 v0 = load8(src), v1 = load8(src + kC);
 #pragma unroll
-            for (int q = 0; q < 8; ++q) dzT[c8 + q][p] = v[q];
+            for (int q = 0; q < 8; ++q) *reinterpret_cast<unsigned*>(&dzT[c8 + q][2 * m]) = pack_bf16x2(v0[q], v1[q]);
         }
-        // stage the input rows: 66 px (x0-1 .. x0+64) of rows y-1..y+1, into the three shifted copies
-        for (int e = tid; e < 3 * 66 * (CIN / 8); e += NT) {
-            const int dyi = e / (66 * (CIN / 8)), rem = e % (66 * (CIN / 8));
-            const int q = rem / (CIN / 8), c8 = (rem % (CIN / 8)) * 8;
+        // stage the input rows: pixel pairs (j, j+1) = (2m, 2m+1), m = 0..32, of rows y-1..y+1
+        for (int e = tid; e < 3 * 33 * (CIN / 8); e += NT) {
+            const int dyi = e / (33 * (CIN / 8)), rem = e % (33 * (CIN / 8));
+            const int m = rem / (CIN / 8), c8 = (rem % (CIN / 8)) * 8;
             const int yy = y + dyi - 1;
-            int xx = x0 - 1 + q;
-            xx = xx < 0 ? xx + W : (xx >= W ? xx - W : xx);
-            bf16x8 v;
-            if (yy >= 0 && yy < H) v = load8(x + (((size_t)b * H + yy) * W + xx) * CIN + c8);
-            else
+            bf16x8 v0, v1;
+            if (yy >= 0 && yy < H) {
+                int xa = x0 - 1 + 2 * m, xb = x0 + 2 * m;
+                xa = xa < 0 ? xa + W : xa;
+                xb = xb >= W ? xb - W : xb;
+                const bf16* row = x + ((size_t)b * H + yy) * W * CIN + c8;
+                v0 = load8(row + (size_t)xa * CIN);
+                v1 = load8(row + (size_t)xb * CIN);
+            } else {
 #pragma unroll
-                for (int qq = 0; qq < 8; ++qq) v[qq] = (bf16)0.0f;
-#pragma unroll
-            for (int dxi = 0; dxi < 3; ++dxi) {
-                const int p = q - dxi;  // copy dx = dxi-1 holds X[x0 + p + dx] at p
-                if (p >= 0 && p < 64)
-#pragma unroll
-                    for (int qq = 0; qq < 8; ++qq) xT[dxi][dyi][c8 + qq][p] = v[qq];
+                for (int q = 0; q < 8; ++q) v0[q] = v1[q] = (bf16)0.0f;
             }
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                *reinterpret_cast<unsigned*>(&xT[dyi][c8 + q][2 * m]) = pack_bf16x2(v0[q], v1[q]);
         }
         __syncthreads();
 #pragma unroll
-        for (int t = 0; t < 9; ++t) {
-            const int dyi = t / 3, dxi = t % 3;
+        for (int dyi = 0; dyi < 3; ++dyi)
 #pragma unroll
             for (int pb = 0; pb < 4; ++pb) {
-                const bf16x8 a = *reinterpret_cast<const bf16x8*>(&dzT[(cb << 5) + i][pb * 16 + kh * 8]);
-                const bf16x8 bb = *reinterpret_cast<const bf16x8*>(&xT[dxi][dyi][(ib << 5) + i][pb * 16 + kh * 8]);
-                acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bb, acc[t], 0, 0, 0);
+                const int p0 = pb * 16 + kh * 8;
+                const bf16x8 a = *reinterpret_cast<const bf16x8*>(&dzT[(cb << 5) + i][p0]);
+                const bf16x8 c0 = *reinterpret_cast<const bf16x8*>(&xT[dyi][(ib << 5) + i][p0]);
+                const bf16x8 c1 = *reinterpret_cast<const bf16x8*>(&xT[dyi][(ib << 5) + i][p0 + 8]);
+                acc[dyi * 3 + 0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, window8<0>(c0, c1), acc[dyi * 3 + 0], 0, 0, 0);
+                acc[dyi * 3 + 1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, window8<1>(c0, c1), acc[dyi * 3 + 1], 0, 0, 0);
+                acc[dyi * 3 + 2] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, window8<2>(c0, c1), acc[dyi * 3 + 2], 0, 0, 0);
             }
-        }
         __syncthreads();
     }
     float* out = partial + (size_t)blockIdx.x * 9 * kC * CIN;
