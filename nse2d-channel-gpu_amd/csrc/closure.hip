@@ -437,9 +437,12 @@ __device__ inline bf16x8 window8(const bf16x8& c0, const bf16x8& c1) {
 // transposed, X^T[dy][ci][j] for the pixels x0 - 1 + j, j = 0..65. Both are staged a pixel pair at
 // a time (two 16-byte loads, eight 4-byte LDS writes). The operand of tap dx starts at j = p + dx + 1:
 // it is cut from two aligned 16-byte LDS reads by funnel shifts, one pair of reads for the three dx.
+// The bias gradient (column sums of dZ) is summed from the staged dZ values on the way; the block's
+// 64 sums go to bias_partial[blockIdx.x].
 template <int CIN>
 __global__ void __launch_bounds__(CIN * 4) conv_wgrad_kernel(const bf16* __restrict__ x, const bf16* __restrict__ dz,
-                                                             float* __restrict__ partial, int H, int W, int B,
+                                                             float* __restrict__ partial,
+                                                             float* __restrict__ bias_partial, int H, int W, int B,
                                                              int segs_per_block) {
     constexpr int NT = CIN * 4;  // threads
     constexpr int LP = 72;       // row length: 66 pixels used, rows 16-byte aligned
@@ -452,6 +455,8 @@ __global__ void __launch_bounds__(CIN * 4) conv_wgrad_kernel(const bf16* __restr
     floatx16 acc[9];
 #pragma unroll
     for (int t = 0; t < 9; ++t) acc[t] = floatx16{0};
+    static_assert(NT % (kC / 8) == 0, "each thread stages the same eight channels of dZ");
+    float bsum[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // channels (tid % 8) * 8 + q
     const int nseg = W >> 6, total = B * H * nseg;
     const int s0 = blockIdx.x * segs_per_block, s1 = min(total, s0 + segs_per_block);
     for (int sg = s0; sg < s1; ++sg) {
@@ -463,7 +468,10 @@ __global__ void __launch_bounds__(CIN * 4) conv_wgrad_kernel(const bf16* __restr
             const bf16* src = dz + (((size_t)b * H + y) * W + x0 + 2 * m) * kC + c8;
             const bf16x8 v0 = load8(src), v1 = load8(src + kC);
 #pragma unroll
-            for (int q = 0; q < 8; ++q) *reinterpret_cast<unsigned*>(&dzT[c8 + q][2 * m]) = pack_bf16x2(v0[q], v1[q]);
+            for (int q = 0; q < 8; ++q) {
+                *reinterpret_cast<unsigned*>(&dzT[c8 + q][2 * m]) = pack_bf16x2(v0[q], v1[q]);
+                bsum[q] += (float)v0[q] + (float)v1[q];
+            }
         }
         // stage the input rows: pixel pairs (j, j+1) = (2m, 2m+1), m = 0..32, of rows y-1..y+1
         for (int e = tid; e < 3 * 33 * (CIN / 8); e += NT) {
@@ -501,6 +509,18 @@ __global__ void __launch_bounds__(CIN * 4) conv_wgrad_kernel(const bf16* __restr
             }
         __syncthreads();
     }
+    {   // bias: the NT / 8 threads of each channel octet, added in a fixed order
+        static_assert(sizeof(xT) >= sizeof(float) * NT * 8, "bias sums fit in the input tile");
+        float* bs = reinterpret_cast<float*>(&xT[0][0][0]);  // [NT / 8][64], free after the last barrier
+#pragma unroll
+        for (int q = 0; q < 8; ++q) bs[(tid >> 3) * kC + (tid & 7) * 8 + q] = bsum[q];
+        __syncthreads();
+        if (tid < kC) {
+            float t = 0.0f;
+            for (int r = 0; r < NT / 8; ++r) t += bs[r * kC + tid];
+            bias_partial[(size_t)blockIdx.x * kC + tid] = t;
+        }
+    }
     float* out = partial + (size_t)blockIdx.x * 9 * kC * CIN;
 #pragma unroll
     for (int t = 0; t < 9; ++t)
@@ -517,7 +537,6 @@ __global__ void __launch_bounds__(CIN * 4) conv_wgrad_kernel(const bf16* __restr
 // interleaved blocks k = g, g + groups, ... and the group sums are added in a fixed order, so the
 // result does not depend on timing.
 constexpr int kSumThreads = 256;
-constexpr int kBiasPixPerBlock = 4096;  // pixels per bias-gradient workgroup
 constexpr int kLastWgradBlocks = 1024;  // most workgroups of the last layer's weight gradient
 __global__ void __launch_bounds__(kSumThreads) sum_partials_kernel(const float* __restrict__ partial, int nb, int n,
                                                                    int cols, float* __restrict__ out) {
@@ -553,36 +572,6 @@ __device__ inline float octet_sum(float v) {
     v += __shfl_xor(v, 16, 64);
     v += __shfl_xor(v, 32, 64);
     return v;
-}
-
-// Bias gradient of a 64-channel layer: per-block partial column sums over a run of pixels. Eight
-// lanes cover a pixel's 64 channels with one 16-byte load each; a workgroup reads 32 pixels a pass.
-__global__ void __launch_bounds__(256) bias_grad_kernel(const bf16* __restrict__ dz, size_t npix, int pix_per_block,
-                                                        float* __restrict__ partial) {
-    __shared__ float sm[4][kC];
-    const int tid = threadIdx.x, sub = tid & 7, grp = tid >> 3, wave = tid >> 6;
-    const size_t p0 = (size_t)blockIdx.x * pix_per_block;
-    const size_t p1 = p0 + pix_per_block < npix ? p0 + pix_per_block : npix;
-    float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    size_t p = p0 + grp;
-    for (; p + 32 < p1; p += 64) {
-        const bf16x8 v0 = load8(dz + p * kC + sub * 8);
-        const bf16x8 v1 = load8(dz + (p + 32) * kC + sub * 8);
-#pragma unroll
-        for (int q = 0; q < 8; ++q) acc[q] += (float)v0[q] + (float)v1[q];
-    }
-    if (p < p1) {
-        const bf16x8 v0 = load8(dz + p * kC + sub * 8);
-#pragma unroll
-        for (int q = 0; q < 8; ++q) acc[q] += (float)v0[q];
-    }
-#pragma unroll
-    for (int q = 0; q < 8; ++q) acc[q] = octet_sum(acc[q]);
-    if ((tid & 63) < 8)
-#pragma unroll
-        for (int q = 0; q < 8; ++q) sm[wave][sub * 8 + q] = acc[q];
-    __syncthreads();
-    if (tid < kC) partial[(size_t)blockIdx.x * kC + tid] = (sm[0][tid] + sm[1][tid]) + (sm[2][tid] + sm[3][tid]);
 }
 
 // The output gradient g of the last layer around a row segment: gs[dyi][j] = g(y + 1 - dyi, x0 - 1 + j)
@@ -890,14 +879,12 @@ void wgrad(CnnImpl* c, const cnn::bf16* x, const cnn::bf16* dz, float* gw, float
     const int nb = std::min(c->wgrad_blocks, total);
     const int per = (total + nb - 1) / nb;
     const int used = (total + per - 1) / per;
-    cnn::conv_wgrad_kernel<CIN><<<used, CIN * 4, 0, c->s>>>(x, dz, c->partial, c->H, c->W, B, per);
+    float* bias_partial = c->partial + (size_t)c->wgrad_blocks * 9 * cnn::kC * cnn::kC;
+    cnn::conv_wgrad_kernel<CIN><<<used, CIN * 4, 0, c->s>>>(x, dz, c->partial, bias_partial, c->H, c->W, B, per);
     NSE2D_LAUNCH_CHECK();
     sum_partials(c, used, 9 * cnn::kC * CIN, gw);
-    const size_t npix = (size_t)B * c->H * c->W;
-    const int nbb = (int)ceil_div(npix, cnn::kBiasPixPerBlock);
-    cnn::bias_grad_kernel<<<nbb, 256, 0, c->s>>>(dz, npix, cnn::kBiasPixPerBlock, c->partial);
+    cnn::sum_partials_kernel<<<ceil_div(cnn::kC, 16), cnn::kSumThreads, 0, c->s>>>(bias_partial, used, cnn::kC, 16, gb);
     NSE2D_LAUNCH_CHECK();
-    sum_partials(c, nbb, cnn::kC, gb);
 }
 
 // Gradients of sum(gout * out) with respect to every parameter, after forward() on the same batch.
@@ -1013,10 +1000,10 @@ CnnImpl* cnn_create(int H, int W, int maxB, unsigned long long seed, hipStream_t
     NSE2D_HIP_CHECK(hipMalloc(&c->out, sizeof(float) * npix));
     NSE2D_HIP_CHECK(hipMalloc(&c->gout, sizeof(float) * npix));
     c->wgrad_blocks = 1024;
-    const size_t pw = (size_t)c->wgrad_blocks * 9 * cnn::kC * cnn::kC;
-    const size_t pb = ceil_div(npix, cnn::kBiasPixPerBlock) * (size_t)cnn::kC;
+    // weight-gradient partials of the hidden layers, then their bias partials
+    const size_t pw = (size_t)c->wgrad_blocks * (9 * cnn::kC * cnn::kC + cnn::kC);
     const size_t pl = (size_t)std::min(row_segments(c, maxB), cnn::kLastWgradBlocks) * (9 * cnn::kC + 1);
-    c->partial_floats = std::max(pw, std::max(pb, pl));
+    c->partial_floats = std::max(pw, pl);
     NSE2D_HIP_CHECK(hipMalloc(&c->partial, sizeof(float) * c->partial_floats));
     NSE2D_HIP_CHECK(hipMalloc(&c->loss_part, sizeof(float) * 1024));
     refresh_bf16(c);
