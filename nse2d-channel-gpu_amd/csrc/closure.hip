@@ -36,6 +36,7 @@ namespace cnn {
 
 typedef __bf16 bf16;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 constexpr int kC = 64;      // hidden channels
@@ -43,11 +44,12 @@ constexpr int kCin0 = 32;   // padded input channels of the first layer
 constexpr int kInputs = 4;  // real input channels
 constexpr int kLayers = 6;
 
-// tanh(u) = 1 - 2 / (1 + e^{2u}) on the native exponential: absolute error ~1e-7, far below the
-// bf16 resolution of the activations it feeds (the GELU below adds it to 1), and a few
-// instructions instead of the library tanhf (the epilogue of every convolution applies it to
-// each output). Saturates correctly: e^{2u} -> inf gives 1, -> 0 gives -1.
-__device__ inline float tanh_fast(float u) { return 1.0f - 2.0f / (1.0f + __expf(2.0f * u)); }
+// tanh(u) = 1 - 2 / (1 + e^{2u}) on the native exponential and reciprocal (v_exp_f32, v_rcp_f32,
+// 1 ulp each): absolute error ~1e-7, far below the bf16 resolution of the activations it feeds
+// (the GELU below adds it to 1), and a few instructions instead of the library tanhf and an IEEE
+// division (the epilogue of every convolution applies it to each output). Saturates correctly:
+// e^{2u} -> inf gives 1, -> 0 gives -1.
+__device__ inline float tanh_fast(float u) { return 1.0f - 2.0f * __builtin_amdgcn_rcpf(1.0f + __expf(2.0f * u)); }
 // GELU, tanh form (Hendrycks and Gimpel): 0.5 z (1 + tanh(s (z + 0.044715 z^3))), s = sqrt(2/pi)
 __device__ inline float gelu(float z) {
     const float s = 0.7978845608028654f;
@@ -238,7 +240,10 @@ struct ConvLdsAll {
 
 // 512 threads: wavefront w computes row y0 + (w & 3), output channels 32 (w >> 2) .. +31, as two
 // 32x32 MFMA tiles (pixels 0..31 and 32..63 of the segment), so that two wavefronts share each
-// SIMD and one's epilogue overlaps the other's MFMAs.
+// SIMD and one's epilogue overlaps the other's MFMAs. The weights are the MFMA's first operand
+// and the pixels its second, so a lane's accumulator holds four consecutive output channels of
+// one pixel in each group of four registers: the epilogue reads zprev and writes zout / aout
+// four channels (8 bytes) at a time.
 constexpr int kAllThreads = 512;
 template <int CIN, int MODE>
 __global__ void __launch_bounds__(kAllThreads) conv_lds_all_kernel(const bf16* __restrict__ in,
@@ -307,8 +312,8 @@ __global__ void __launch_bounds__(kAllThreads) conv_lds_all_kernel(const bf16* _
                 const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(&sh.in[lr][i + 1 + dx][co]);
                 const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(&sh.in[lr][i + 33 + dx][co]);
                 const bf16x8 bw = *reinterpret_cast<const bf16x8*>(&sh.w[t][(cb << 5) + i][co]);
-                acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, bw, acc[0], 0, 0, 0);
-                acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, bw, acc[1], 0, 0, 0);
+                acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bw, a0, acc[0], 0, 0, 0);
+                acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bw, a1, acc[1], 0, 0, 0);
             }
         }
         const int nxt = tile + gridDim.x;
@@ -316,22 +321,32 @@ __global__ void __launch_bounds__(kAllThreads) conv_lds_all_kernel(const bf16* _
         const int seg = tile % nseg, yt = (tile / nseg) % nyt, b = tile / (nseg * nyt);
         const int x0 = seg << 6, y = yt * kTileRows + wrow;
         if (y < H) {
+            // acc[pb][4 g + j]: pixel x0 + 32 pb + i, channel 32 cb + 8 g + 4 kh + j
 #pragma unroll
-            for (int pb = 0; pb < 2; ++pb)
+            for (int pb = 0; pb < 2; ++pb) {
+                const size_t pix = ((size_t)b * H + y) * W + x0 + (pb << 5) + i;
 #pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    const int px = x0 + (pb << 5) + ((r >> 2) << 3) + (kh << 2) + (r & 3);
-                    const int oc = (cb << 5) + i;
-                    const size_t o = (((size_t)b * H + y) * W + px) * kC + oc;
+                for (int g = 0; g < 4; ++g) {
+                    const int oc0 = (cb << 5) + (g << 3) + (kh << 2);
+                    const size_t o = pix * kC + oc0;
+                    bf16x4 ov;
                     if (MODE == MODE_BWD_DATA) {
-                        aout[o] = (bf16)(acc[pb][r] * gelu_grad((float)zprev[o]));
+                        const bf16x4 zp = *reinterpret_cast<const bf16x4*>(zprev + o);
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) ov[j] = (bf16)(acc[pb][4 * g + j] * gelu_grad((float)zp[j]));
                     } else {
-                        const float z = acc[pb][r] + bias[oc];
-                        const bf16 zb = (bf16)z;
-                        if (zout) zout[o] = zb;
-                        if (MODE == MODE_FWD_GELU) aout[o] = (bf16)gelu((float)zb);
+                        bf16x4 zv;
+#pragma unroll
+                        for (int j = 0; j < 4; ++j) {
+                            zv[j] = (bf16)(acc[pb][4 * g + j] + bias[oc0 + j]);
+                            if (MODE == MODE_FWD_GELU) ov[j] = (bf16)gelu((float)zv[j]);
+                        }
+                        if (zout) *reinterpret_cast<bf16x4*>(zout + o) = zv;
+                        if (MODE != MODE_FWD_GELU) continue;
                     }
+                    *reinterpret_cast<bf16x4*>(aout + o) = ov;
                 }
+            }
         }
         __syncthreads();  // every wavefront is done with sh.in before the next deposit
     }
