@@ -1014,7 +1014,9 @@ CnnImpl* cnn_create(int H, int W, int maxB, unsigned long long seed, hipStream_t
     NSE2D_HIP_CHECK(hipMalloc(&c->dz1, sizeof(cnn::bf16) * npix * cnn::kC));
     NSE2D_HIP_CHECK(hipMalloc(&c->out, sizeof(float) * npix));
     NSE2D_HIP_CHECK(hipMalloc(&c->gout, sizeof(float) * npix));
-    c->wgrad_blocks = 1024;
+    // two weight-gradient workgroups are resident per CU (37 KB of LDS, 200 VGPRs): one round of
+    // them, each over a long run of row segments, keeps the partial sums small
+    c->wgrad_blocks = 2 * c->cus;
     // weight-gradient partials of the hidden layers, then their bias partials
     const size_t pw = (size_t)c->wgrad_blocks * (9 * cnn::kC * cnn::kC + cnn::kC);
     const size_t pl = (size_t)std::min(row_segments(c, maxB), cnn::kLastWgradBlocks) * (9 * cnn::kC + 1);
