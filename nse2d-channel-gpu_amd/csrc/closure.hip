@@ -238,7 +238,7 @@ struct ConvLdsAll {
     bf16 in[kTileRows + 2][66][CP];
 };
 
-// 512 threads: wavefront w computes row y0 + (w & 3), output channels 32 (w >> 2) .. +31, as two
+// 512 threads: wavefront w computes row y0 + (w >> 1), output channels 32 (w & 1) .. +31, as two
 // 32x32 MFMA tiles (pixels 0..31 and 32..63 of the segment), so that two wavefronts share each
 // SIMD and one's epilogue overlaps the other's MFMAs. The weights are the MFMA's first operand
 // and the pixels its second, so a lane's accumulator holds four consecutive output channels of
@@ -258,7 +258,9 @@ __global__ void __launch_bounds__(kAllThreads) conv_lds_all_kernel(const bf16* _
     constexpr int NIN = ((kTileRows + 2) * 66 * C8 + NT - 1) / NT;  // input chunks per thread
     __shared__ alignas(16) S sh;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wrow = wave & 3, cb = wave >> 2;
+    // the two wavefronts of a row sit on different SIMDs, so that the last row tile of an odd
+    // height (rows past the field skip their products) takes half the time of a full one
+    const int wrow = wave >> 1, cb = wave & 1;
     const int nseg = W >> 6, nyt = (H + kTileRows - 1) / kTileRows;
     const int i = lane & 31, kh = lane >> 5;
     for (int e = tid; e < 9 * kC * C8; e += NT) {
@@ -301,19 +303,22 @@ __global__ void __launch_bounds__(kAllThreads) conv_lds_all_kernel(const bf16* _
         floatx16 acc[2];
         acc[0] = floatx16{0};
         acc[1] = floatx16{0};
+        const bool row_in = ((tile / nseg) % nyt) * kTileRows + wrow < H;  // uniform over the wavefront
+        if (row_in) {
 #pragma unroll
-        for (int t = 0; t < 9; ++t) {
-            int dy = t / 3 - 1, dx = t % 3 - 1;
-            if (MODE == MODE_BWD_DATA) { dy = -dy; dx = -dx; }
-            const int lr = wrow + 1 + dy;
+            for (int t = 0; t < 9; ++t) {
+                int dy = t / 3 - 1, dx = t % 3 - 1;
+                if (MODE == MODE_BWD_DATA) { dy = -dy; dx = -dx; }
+                const int lr = wrow + 1 + dy;
 #pragma unroll
-            for (int kb = 0; kb < CIN / 16; ++kb) {
-                const int co = kb * 16 + kh * 8;
-                const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(&sh.in[lr][i + 1 + dx][co]);
-                const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(&sh.in[lr][i + 33 + dx][co]);
-                const bf16x8 bw = *reinterpret_cast<const bf16x8*>(&sh.w[t][(cb << 5) + i][co]);
-                acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bw, a0, acc[0], 0, 0, 0);
-                acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bw, a1, acc[1], 0, 0, 0);
+                for (int kb = 0; kb < CIN / 16; ++kb) {
+                    const int co = kb * 16 + kh * 8;
+                    const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(&sh.in[lr][i + 1 + dx][co]);
+                    const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(&sh.in[lr][i + 33 + dx][co]);
+                    const bf16x8 bw = *reinterpret_cast<const bf16x8*>(&sh.w[t][(cb << 5) + i][co]);
+                    acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bw, a0, acc[0], 0, 0, 0);
+                    acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bw, a1, acc[1], 0, 0, 0);
+                }
             }
         }
         const int nxt = tile + gridDim.x;
