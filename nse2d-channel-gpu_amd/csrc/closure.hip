@@ -896,7 +896,10 @@ int row_segments(const CnnImpl* c, int B) { return B * c->H * (c->W / 64); }
 template <int CIN>
 void wgrad(CnnImpl* c, const cnn::bf16* x, const cnn::bf16* dz, float* gw, float* gb, int B) {
     const int total = B * c->H * (c->W / 64);
-    const int nb = std::min(c->wgrad_blocks, total);
+    // one round of resident workgroups, each over a long run of row segments (keeps the partial
+    // sums small): at about 200 VGPRs a SIMD holds two wavefronts, so a CU holds 8 / (CIN / 16)
+    // workgroups of CIN / 16 wavefronts (two at CIN = 64, four at CIN = 32)
+    const int nb = std::min(std::min(c->wgrad_blocks, c->cus * (128 / CIN)), total);
     const int per = (total + nb - 1) / nb;
     const int used = (total + per - 1) / per;
     float* bias_partial = c->partial + (size_t)c->wgrad_blocks * 9 * cnn::kC * cnn::kC;
@@ -1019,9 +1022,8 @@ CnnImpl* cnn_create(int H, int W, int maxB, unsigned long long seed, hipStream_t
     NSE2D_HIP_CHECK(hipMalloc(&c->dz1, sizeof(cnn::bf16) * npix * cnn::kC));
     NSE2D_HIP_CHECK(hipMalloc(&c->out, sizeof(float) * npix));
     NSE2D_HIP_CHECK(hipMalloc(&c->gout, sizeof(float) * npix));
-    // two weight-gradient workgroups are resident per CU (37 KB of LDS, 200 VGPRs): one round of
-    // them, each over a long run of row segments, keeps the partial sums small
-    c->wgrad_blocks = 2 * c->cus;
+    // most weight-gradient workgroups of a launch (see wgrad)
+    c->wgrad_blocks = 4 * c->cus;
     // weight-gradient partials of the hidden layers, then their bias partials
     const size_t pw = (size_t)c->wgrad_blocks * (9 * cnn::kC * cnn::kC + cnn::kC);
     const size_t pl = (size_t)std::min(row_segments(c, maxB), cnn::kLastWgradBlocks) * (9 * cnn::kC + 1);
