@@ -18,6 +18,7 @@
 // dimension as K: tiles of 64 pixels are staged transposed ([channel][pixel]) in LDS, with the
 // three x-shifted copies of the input rows, so that each lane reads 8 consecutive pixels.
 #include "nse2d_common.hpp"
+#include "nse2d_cnn.hpp"
 #include "../../include/nse2d_hip.h"
 
 #include <algorithm>
@@ -363,12 +364,14 @@ __global__ void __launch_bounds__(kAllThreads) conv_lds_all_kernel(const bf16* _
 // of the 66 pixels the segment reaches, eight lanes (eight channels each, one 16-byte load) form
 // the row's three partial sums s[dy][dx][px], reduced over the lanes and kept in LDS; the 64
 // outputs then add nine of them. Each input pixel is read once per output row that needs it,
-// instead of once per tap.
+// instead of once per tap. closure != 0: the closure term of a coarse channel is written instead,
+// (out) * scale + shift on rows 1..H-2 and 0 on the wall rows.
 constexpr int kLastThreads = 256;
 __global__ void __launch_bounds__(kLastThreads) last_fwd_rows_kernel(const bf16* __restrict__ a,
                                                                     const float* __restrict__ w,
                                                                     const float* __restrict__ bias,
-                                                                    float* __restrict__ out, int H, int W) {
+                                                                    float* __restrict__ out, int H, int W,
+                                                                    int closure, float scale, float shift) {
     __shared__ float sp[3][3][68];
     const int nseg = W >> 6;
     const int seg = blockIdx.x % nseg, y = (blockIdx.x / nseg) % H, b = blockIdx.x / (nseg * H);
@@ -419,7 +422,9 @@ __global__ void __launch_bounds__(kLastThreads) last_fwd_rows_kernel(const bf16*
         for (int r = 0; r < 3; ++r)
 #pragma unroll
             for (int k = 0; k < 3; ++k) o += sp[r][k][tid + k];
-        out[((size_t)b * H + y) * W + x0 + tid] = o + bias[0];
+        o += bias[0];
+        if (closure) o = (y > 0 && y < H - 1) ? o * scale + shift : 0.0f;
+        out[((size_t)b * H + y) * W + x0 + tid] = o;
     }
 }
 
@@ -868,19 +873,26 @@ void launch_conv(CnnImpl* c, const cnn::bf16* in, const cnn::bf16* w, const floa
 }
 
 // keep = false (inference): the pre-activations z are not stored, so no backward may follow.
-// out: where the network's output goes (nullptr: the model's own buffer, which the loss reads)
-void forward(CnnImpl* c, const float* d_in, int B, bool keep = true, float* out = nullptr) {
-    const size_t npix = (size_t)B * c->H * c->W;
-    cnn::pack_input_kernel<<<ceil_div(npix, 256), 256, 0, c->s>>>(d_in, c->x0, npix);
-    NSE2D_LAUNCH_CHECK();
+// The layers after the input packing. out: where the network's output goes (nullptr: the model's
+// own buffer, which the loss reads); closure: the last layer writes the closure term instead
+// (see last_fwd_rows_kernel).
+void forward_packed(CnnImpl* c, int B, bool keep, float* out, int closure = 0, float scale = 1.0f,
+                    float shift = 0.0f) {
     launch_conv<cnn::kCin0, cnn::MODE_FWD_GELU>(c, c->x0, c->wb[0], c->p + c->off_b[0], keep ? c->z[0] : nullptr,
                                                 c->a[0], nullptr, B);
     for (int l = 1; l < 5; ++l)
         launch_conv<cnn::kC, cnn::MODE_FWD_GELU>(c, c->a[l - 1], c->wb[l], c->p + c->off_b[l],
                                                  keep ? c->z[l] : nullptr, c->a[l], nullptr, B);
     cnn::last_fwd_rows_kernel<<<B * c->H * (c->W / 64), cnn::kLastThreads, 0, c->s>>>(
-        c->a[4], c->p + c->off_w[5], c->p + c->off_b[5], out ? out : c->out, c->H, c->W);
+        c->a[4], c->p + c->off_w[5], c->p + c->off_b[5], out ? out : c->out, c->H, c->W, closure, scale, shift);
     NSE2D_LAUNCH_CHECK();
+}
+
+void forward(CnnImpl* c, const float* d_in, int B, bool keep = true, float* out = nullptr) {
+    const size_t npix = (size_t)B * c->H * c->W;
+    cnn::pack_input_kernel<<<ceil_div(npix, 256), 256, 0, c->s>>>(d_in, c->x0, npix);
+    NSE2D_LAUNCH_CHECK();
+    forward_packed(c, B, keep, out);
 }
 
 // out[j] = sum over nb blocks of partial[k][j], j < n
@@ -1018,6 +1030,8 @@ CnnImpl* cnn_create(int H, int W, int maxB, unsigned long long seed, hipStream_t
         NSE2D_HIP_CHECK(hipMalloc(&c->a[l], sizeof(cnn::bf16) * npix * cnn::kC));
     }
     NSE2D_HIP_CHECK(hipMalloc(&c->x0, sizeof(cnn::bf16) * npix * cnn::kCin0));
+    // channels kInputs..kCin0-1 stay zero (cnn_packed_input)
+    NSE2D_HIP_CHECK(hipMemsetAsync(c->x0, 0, sizeof(cnn::bf16) * npix * cnn::kCin0, s));
     NSE2D_HIP_CHECK(hipMalloc(&c->dz0, sizeof(cnn::bf16) * npix * cnn::kC));
     NSE2D_HIP_CHECK(hipMalloc(&c->dz1, sizeof(cnn::bf16) * npix * cnn::kC));
     NSE2D_HIP_CHECK(hipMalloc(&c->out, sizeof(float) * npix));
@@ -1063,6 +1077,13 @@ void cnn_adam(CnnImpl* c, float lr) {
 struct nse2d_cnn {
     nse2d::CnnImpl* impl;
 };
+
+namespace nse2d {
+__bf16* cnn_packed_input(nse2d_cnn_t c) { return c->impl->x0; }
+void cnn_infer_packed_closure(nse2d_cnn_t c, float scale, float shift, float* dst) {
+    forward_packed(c->impl, 1, false, dst, 1, scale, shift);
+}
+}  // namespace nse2d
 
 using namespace nse2d;
 

@@ -23,6 +23,7 @@
 // CNN inputs: (omega, psi, u, v) of the coarse state, u and v as ops.cu:47-63 (clamped wall
 // differences), each normalised by its mean and standard deviation over the training set; the
 // CNN predicts the normalised target.
+#include "nse2d_cnn.hpp"
 #include "nse2d_common.hpp"
 #include "nse2d_poisson.hpp"
 #include "../../include/nse2d_hip.h"
@@ -82,9 +83,12 @@ __global__ void truncate_kernel(const double2* __restrict__ hf, double2* __restr
 // CNN inputs of a coarse state, [Ny][Nx][4] fp32, each channel as (value - mean) * inv_std;
 // u = a_j (psi_{j+1} - psi_{j-1}) / (2 d_eta), v = -(psi_{i+1} - psi_{i-1}) / (2 dx) with the
 // wall rows' differences clamped to the row itself (ops.cu:47-63).
+// PACKED: out is the network's packed bf16 input ([Ny][Nx][32]); the inputs go to channels 0..3
+// as bf16 of their fp32 values, as the network's own packing rounds them.
+template <bool PACKED, typename OutT>
 __global__ void features_kernel(const double* __restrict__ w, const double* __restrict__ psi, int Nx, int Ny,
                                 double inv_dx, double inv_deta, const double* __restrict__ a_node,
-                                float* __restrict__ out, Norm n) {
+                                OutT* __restrict__ out, Norm n) {
     const int ix = blockIdx.x * blockDim.x + threadIdx.x, iy = blockIdx.y;
     if (ix >= Nx) return;
     const int xm = wrap_x(ix - 1, Nx), xp = wrap_x(ix + 1, Nx);
@@ -98,7 +102,13 @@ __global__ void features_kernel(const double* __restrict__ w, const double* __re
     o.y = (float)((val[1] - n.mean[1]) * n.inv_std[1]);
     o.z = (float)((val[2] - n.mean[2]) * n.inv_std[2]);
     o.w = (float)((val[3] - n.mean[3]) * n.inv_std[3]);
-    reinterpret_cast<float4*>(out)[c] = o;
+    if constexpr (PACKED) {
+        typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+        const bf16x4 p = {(__bf16)o.x, (__bf16)o.y, (__bf16)o.z, (__bf16)o.w};
+        *reinterpret_cast<bf16x4*>(out + c * 32) = p;
+    } else {
+        reinterpret_cast<float4*>(out)[c] = o;
+    }
 }
 
 // target = rf - rc on the interior rows, 0 on the wall rows; fp64 copy and/or fp32 copy.
@@ -170,15 +180,6 @@ __global__ void __launch_bounds__(256) zero_loss_kernel(const float* __restrict_
     if (threadIdx.x == 0) { partial[2 * blockIdx.x] = acc[0]; partial[2 * blockIdx.x + 1] = acc[1]; }
 }
 
-// Closure term from the CNN output: (out * std + mean) on rows 1..Ny-2, 0 on the wall rows.
-__global__ void closure_scale_kernel(const float* __restrict__ out, int Nx, int Ny, float mean, float std,
-                                     float* __restrict__ dst) {
-    const int ix = blockIdx.x * blockDim.x + threadIdx.x, iy = blockIdx.y;
-    if (ix >= Nx) return;
-    const size_t c = idx2(iy, ix, Nx);
-    dst[c] = (iy > 0 && iy < Ny - 1) ? out[c] * std + mean : 0.0f;
-}
-
 struct Grid {
     nse2d_channel_desc d;
     double inv_dx = 0, inv_deta = 0, inv_dx2 = 0, inv_deta2 = 0;
@@ -208,7 +209,6 @@ struct LesImpl {
     int max_pairs = 0, npairs = 0, max_batch = 0;
     float *X = nullptr, *Y = nullptr;            // raw training set [max_pairs][Ny_c][Nx_c]([4])
     float *Xb = nullptr, *Yb = nullptr;          // normalised batch
-    float* out = nullptr;                        // CNN output [Ny_c][Nx_c]
     int* d_idx = nullptr;
     Norm norm{};
     double mean[kFeat + 1] = {0}, stdv[kFeat + 1] = {1, 1, 1, 1, 1};
@@ -251,8 +251,8 @@ void rhs(LesImpl* L, const Grid& g, const double* w, const double* psi, double* 
 
 void features(LesImpl* L, const double* w, const double* psi, float* out, const Norm& n) {
     const Grid& g = L->c;
-    features_kernel<<<dim3(ceil_div(g.d.Nx, 256), g.d.Ny), 256, 0, L->s>>>(w, psi, g.d.Nx, g.d.Ny, g.inv_dx,
-                                                                         g.inv_deta, g.d.d_a_node, out, n);
+    features_kernel<false><<<dim3(ceil_div(g.d.Nx, 256), g.d.Ny), 256, 0, L->s>>>(
+        w, psi, g.d.Nx, g.d.Ny, g.inv_dx, g.inv_deta, g.d.d_a_node, out, n);
     NSE2D_LAUNCH_CHECK();
 }
 
@@ -269,12 +269,12 @@ void closure_hook(void* ctx, const double* w, const double* psi, float* dst, hip
     // mode 2: the channel calls the hook before each of the three stages of a step; only the
     // first call of a step evaluates, and the channel's buffer keeps that term for the others
     if (L->attach_mode == 2 && (L->hook_calls++ % 3) != 0) return;
-    features(L, w, psi, L->Xb, L->norm);
-    nse2d_cnn_infer(L->cnn, L->Xb, L->out, 1);
+    // the inputs straight into the network's packed input, the closure term straight into dst
     const Grid& g = L->c;
-    closure_scale_kernel<<<dim3(ceil_div(g.d.Nx, 256), g.d.Ny), 256, 0, L->s>>>(
-        L->out, g.d.Nx, g.d.Ny, (float)L->mean[kFeat], (float)L->stdv[kFeat], dst);
+    features_kernel<true><<<dim3(ceil_div(g.d.Nx, 256), g.d.Ny), 256, 0, L->s>>>(
+        w, psi, g.d.Nx, g.d.Ny, g.inv_dx, g.inv_deta, g.d.d_a_node, cnn_packed_input(L->cnn), L->norm);
     NSE2D_LAUNCH_CHECK();
+    cnn_infer_packed_closure(L->cnn, (float)L->stdv[kFeat], (float)L->mean[kFeat], dst);
 }
 
 std::vector<double> read_partials(LesImpl* L, size_t n) {
@@ -289,7 +289,7 @@ void les_destroy(LesImpl* L) {
     (void)hipStreamSynchronize(L->s);
     for (void* p : {(void*)L->tmp, (void*)L->fJ, (void*)L->flap, (void*)L->frhs, (void*)L->cw, (void*)L->cpsi,
                     (void*)L->cJ, (void*)L->clap, (void*)L->crhs, (void*)L->cfr, (void*)L->partial, (void*)L->X,
-                    (void*)L->Y, (void*)L->Xb, (void*)L->Yb, (void*)L->out, (void*)L->d_idx})
+                    (void*)L->Y, (void*)L->Xb, (void*)L->Yb, (void*)L->d_idx})
         (void)hipFree(p);
     if (L->Pf) poisson_destroy(L->Pf);
     if (L->Pc) poisson_destroy(L->Pc);
@@ -338,7 +338,6 @@ LesImpl* les_create(const nse2d_channel_desc* fine, const nse2d_channel_desc* co
     NSE2D_HIP_CHECK(hipMalloc(&L->Y, sizeof(float) * L->c.N * max_pairs));
     NSE2D_HIP_CHECK(hipMalloc(&L->Xb, sizeof(float) * kFeat * L->c.N * max_batch));
     NSE2D_HIP_CHECK(hipMalloc(&L->Yb, sizeof(float) * L->c.N * max_batch));
-    NSE2D_HIP_CHECK(hipMalloc(&L->out, sizeof(float) * L->c.N));
     NSE2D_HIP_CHECK(hipMalloc(&L->d_idx, sizeof(int) * max_batch));
     set_norm(L);
     L->cnn = nse2d_cnn_create(C.Ny, C.Nx, max_batch, seed, s);
