@@ -235,6 +235,10 @@ typedef struct {
     double cfl, cvisc, dt_max, dy_min;
     /* borrowed device metric arrays */
     const double *d_a_node, *d_a_edge, *d_y_node, *d_dy_edge, *d_w_node;
+    /* wall closure of this channel: 1 = no-slip Thom closure (ops.cu:112-156), 2 = free-slip
+     * (omega = 0 on the walls, the FREE_SLIP_WALLS build of ops.cu:108-111), 0 = the library-wide
+     * default of nse2d_set_wall_type, read at each step (an ensemble reads it once, at creation) */
+    int wall_type;
 } nse2d_channel_desc;
 
 typedef struct nse2d_channel* nse2d_channel_t;
@@ -256,6 +260,8 @@ void nse2d_channel_step(nse2d_channel_t ch, double t, double dt, int flags);
 /* n steps with dt chosen on the device by the rule of main.cu:440-458 (no host sync; the
  * current dt and t live in device memory). t advances from its device value. */
 void nse2d_channel_step_auto(nse2d_channel_t ch, int n);
+/* The stream the channel was created with (all of its work runs on it). */
+hipStream_t nse2d_channel_stream(nse2d_channel_t ch);
 /* Device-resident time and dt (read/write through these device pointers). */
 double* nse2d_channel_time_ptr(nse2d_channel_t ch);
 double* nse2d_channel_dt_ptr(nse2d_channel_t ch);
@@ -397,9 +403,11 @@ int nse2d_les_eval(nse2d_les_t L, int first, int count, double* loss_cnn, double
 /* The closure term of one coarse state into d_out ([Ny_c][Nx_c] fp32, 0 on the wall rows). */
 int nse2d_les_closure(nse2d_les_t L, const double* d_omega_c, const double* d_psi_c, float* d_out);
 /* on = 1: the coarse channel adds the closure of each stage state to its RHS (section 5,
- * nse2d_channel_set_closure); on = 2: the closure of each step's first stage state is evaluated
- * once and added in all three stages (a third of the network evaluations; the closure term is
- * then first order in time); on = 0 detaches it. 0 on success. */
+ * nse2d_channel_set_closure); on = 2: the closure of each step's first stage state (the
+ * channel's omega^n field) is evaluated once and added in all three stages (a third of the
+ * network evaluations; the closure term is then first order in time); on = 0 detaches it.
+ * The channel must have been created on the stream given to nse2d_les_create (else -1).
+ * nse2d_les_closure always evaluates, whatever is attached. 0 on success. */
 int nse2d_les_attach(nse2d_les_t L, nse2d_channel_t coarse_ch, int on);
 
 /* ------------------------------------------------------------------------------------------
@@ -412,6 +420,10 @@ void nse2d_memcpy_h2d(void* dst, const void* src, size_t bytes);
 void nse2d_memcpy_d2h(void* dst, const void* src, size_t bytes);
 void nse2d_memcpy_d2d(void* dst, const void* src, size_t bytes);
 void nse2d_device_sync(void);
+/* A new stream of default priority, its destruction, and a wait for its work. */
+hipStream_t nse2d_stream_create(void);
+void nse2d_stream_destroy(hipStream_t s);
+void nse2d_stream_sync(hipStream_t s);
 int nse2d_device_count(void);
 const char* nse2d_last_error(void);
 const char* nse2d_version(void);

@@ -631,6 +631,9 @@ struct ChannelImpl {
     double* dt_ptr() { return scalars + 1; }
     double* max_ptr() { return scalars + 2; }
     double* cum_ptr() { return scalars + 5; }
+    // wall closure of this channel (0: Thom, 1: free-slip): its descriptor's choice, else the
+    // library-wide default at the time of the call
+    int wall() const { return d.wall_type == 1 ? 0 : d.wall_type == 2 ? 1 : wall_type(); }
 };
 
 namespace {
@@ -752,17 +755,17 @@ void close_stage(ChannelImpl* c, double* f, int stage, bool bookkeeping, bool fo
     if (!bookkeeping && poisson_fuses_closure(c->P)) {
         // the closure rides in the inverse transform's workgroups of rows 1 and Ny-2
         ProfScope ps(c, NSE2D_PROF_FFT_INV);
-        poisson_inverse_closure(c->P, c->psi, f, d.d_dy_edge, d.psi_bot, d.psi_top, wall_type(), c->s);
+        poisson_inverse_closure(c->P, c->psi, f, d.d_dy_edge, d.psi_bot, d.psi_top, c->wall(), c->s);
         return;
     }
     { ProfScope ps(c, NSE2D_PROF_FFT_INV); poisson_inverse(c->P, c->psi, c->s); }
     ProfScope ps(c, NSE2D_PROF_THOM);
     if (bookkeeping) {
         thom_bookkeeping_kernel<<<1, 1024, 0, c->s>>>(f, c->psi, d.Nx, d.Ny, d.d_dy_edge, d.d_w_node, d.psi_bot,
-                                                      d.psi_top, wall_type(), c->cum_ptr() + stage);
+                                                      d.psi_top, c->wall(), c->cum_ptr() + stage);
         NSE2D_LAUNCH_CHECK();
     } else {
-        launch_thom(f, c->psi, d.Nx, d.Ny, d.d_dy_edge, d.psi_bot, d.psi_top, wall_type(), c->s);
+        launch_thom(f, c->psi, d.Nx, d.Ny, d.d_dy_edge, d.psi_bot, d.psi_top, c->wall(), c->s);
     }
 }
 
@@ -844,6 +847,7 @@ namespace {
 
 struct EnsClass {
     int Nx = 0, Ny = 0;
+    int wall = 0;  // 0: Thom closure, 1: free-slip (resolved when the ensemble is created)
     double Lx = 0, psi_bot = 0, psi_top = 0;
     const double *a_node = nullptr, *a_edge = nullptr, *dy_edge = nullptr;
     PoissonImpl* P = nullptr;
@@ -858,6 +862,7 @@ struct EnsMember {
     double *w = nullptr, *w1 = nullptr, *w2 = nullptr, *psi = nullptr, *f_row = nullptr;
     double dx = 0, deta = 0, inv_dx = 0, inv_deta = 0, inv_dx2 = 0, inv_deta2 = 0;
     int cls = 0;
+    int wall = 0;
 };
 
 int stage_code_ry(int c) { return (c % 10000) / 100; }
@@ -971,7 +976,7 @@ void ens_close(EnsembleImpl* e, int field) {
         poisson_forward_group(c.P, tab + (size_t)field * nc, c.d_hats, nc, e->s);
         poisson_tridiag_group(c.P, c.d_hats, nc, c.psi_bot, c.psi_top, e->s);
         poisson_inverse_group(c.P, c.d_hats, tab + (size_t)3 * nc, tab + (size_t)field * nc, nc, c.dy_edge,
-                              c.psi_bot, c.psi_top, wall_type(), e->s);
+                              c.psi_bot, c.psi_top, c.wall, e->s);
     }
 }
 
@@ -1037,6 +1042,10 @@ EnsembleImpl* ens_create(const nse2d_channel_desc* descs, int n, hipStream_t s) 
             set_last_error("nse2d_ensemble_create: member " + std::to_string(m) + " metric arrays must be set");
             return nullptr;
         }
+        if (d.wall_type < 0 || d.wall_type > 2) {
+            set_last_error("nse2d_ensemble_create: member " + std::to_string(m) + " wall_type must be 0, 1 or 2");
+            return nullptr;
+        }
     }
     auto* e = new EnsembleImpl;
     e->s = s;
@@ -1049,16 +1058,18 @@ EnsembleImpl* ens_create(const nse2d_channel_desc* descs, int n, hipStream_t s) 
         EnsMember& M = e->mem[m];
         M.d = descs[m];
         const nse2d_channel_desc& d = M.d;
+        const int wall = d.wall_type == 1 ? 0 : d.wall_type == 2 ? 1 : wall_type();
+        M.wall = wall;
         int k = 0;
         for (; k < (int)e->cls.size(); ++k) {
             const EnsClass& c = e->cls[k];
             if (c.Nx == d.Nx && c.Ny == d.Ny && c.Lx == d.Lx && c.a_node == d.d_a_node && c.a_edge == d.d_a_edge &&
-                c.dy_edge == d.d_dy_edge && c.psi_bot == d.psi_bot && c.psi_top == d.psi_top)
+                c.dy_edge == d.d_dy_edge && c.psi_bot == d.psi_bot && c.psi_top == d.psi_top && c.wall == wall)
                 break;
         }
         if (k == (int)e->cls.size()) {
             EnsClass c;
-            c.Nx = d.Nx; c.Ny = d.Ny; c.Lx = d.Lx; c.psi_bot = d.psi_bot; c.psi_top = d.psi_top;
+            c.Nx = d.Nx; c.Ny = d.Ny; c.Lx = d.Lx; c.psi_bot = d.psi_bot; c.psi_top = d.psi_top; c.wall = wall;
             c.a_node = d.d_a_node; c.a_edge = d.d_a_edge; c.dy_edge = d.d_dy_edge;
             c.P = poisson_create_with(d.Nx, d.Ny, d.Lx, -1, -1, e->s);
             if (!c.P) { ens_destroy(e); return nullptr; }
@@ -1164,7 +1175,7 @@ void ens_init(EnsembleImpl* e) {
                               0, e->s);
     }
     for (EnsMember& M : e->mem)
-        launch_thom(M.w, M.psi, M.d.Nx, M.d.Ny, M.d.d_dy_edge, M.d.psi_bot, M.d.psi_top, wall_type(), e->s);
+        launch_thom(M.w, M.psi, M.d.Nx, M.d.Ny, M.d.d_dy_edge, M.d.psi_bot, M.d.psi_top, M.wall, e->s);
 }
 
 // Mean ms per step of nsteps un-captured steps after 2 warm-up steps.
@@ -1282,6 +1293,10 @@ nse2d_channel_t nse2d_channel_create(const nse2d_channel_desc* desc, hipStream_t
         set_last_error("nse2d_channel_create: metric arrays must be set");
         return nullptr;
     }
+    if (d.wall_type < 0 || d.wall_type > 2) {
+        set_last_error("nse2d_channel_create: wall_type must be 0 (library default), 1 (no-slip) or 2 (free-slip)");
+        return nullptr;
+    }
     PoissonImpl* P = poisson_create(d.Nx, d.Ny, d.Lx, s);
     if (!P) return nullptr;
     auto* ch = new nse2d_channel;
@@ -1345,7 +1360,7 @@ void nse2d_channel_init(nse2d_channel_t ch) {
     ChannelImpl* c = &ch->impl;
     const nse2d_channel_desc& d = c->d;
     poisson_solve(c->P, c->w, c->psi, d.psi_bot, d.psi_top, c->s);
-    launch_thom(c->w, c->psi, d.Nx, d.Ny, d.d_dy_edge, d.psi_bot, d.psi_top, wall_type(), c->s);
+    launch_thom(c->w, c->psi, d.Nx, d.Ny, d.d_dy_edge, d.psi_bot, d.psi_top, c->wall(), c->s);
 }
 
 void nse2d_channel_step(nse2d_channel_t ch, double t, double dt, int flags) {
@@ -1421,6 +1436,8 @@ void nse2d_channel_profile_read(nse2d_channel_t ch, double* ms, long long* count
         if (count) count[k] = c->prof_n[k];
     }
 }
+
+hipStream_t nse2d_channel_stream(nse2d_channel_t ch) { return ch->impl.s; }
 
 int nse2d_channel_set_closure(nse2d_channel_t ch, nse2d_closure_fn fn, void* ctx) {
     ChannelImpl* c = &ch->impl;

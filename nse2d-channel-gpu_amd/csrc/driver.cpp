@@ -555,7 +555,6 @@ int run(Run& R) {
             meanFx_A += P.F0 * std::sin((double)P.nforce * kPi * ((G.y[j] + G.h) / (2.0 * G.h))) * G.w_node[j];
         meanFx_A /= Ly_full;
     }
-    nse2d_set_wall_type(P.free_slip ? 1 : 0);
 
     double* d_a_node = dalloc<double>(Ny);
     double* d_a_edge = dalloc<double>(Ny - 1);
@@ -590,6 +589,7 @@ int run(Run& R) {
     cd.mms = P.mms ? 1 : 0; cd.mms_A0 = P.mms_A0; cd.mms_eps = P.mms_eps; cd.mms_Om = P.mms_Om; cd.mms_kx = P.mms_kx;
     cd.cfl = P.cfl; cd.cvisc = P.cvisc; cd.dt_max = P.dt_max; cd.dy_min = dy_min;
     cd.d_a_node = d_a_node; cd.d_a_edge = d_a_edge; cd.d_y_node = d_y_node; cd.d_dy_edge = d_dy_edge; cd.d_w_node = d_w_node;
+    cd.wall_type = P.free_slip ? 2 : 1;  // this run's closure; the library-wide default is left alone
     nse2d_channel_t ch = nse2d_channel_create(&cd, stream);
     if (!ch) { std::fprintf(stderr, "ERROR: %s\n", nse2d_last_error()); return 1; }
     double* d_w = nse2d_channel_field(ch, NSE2D_FIELD_OMEGA);

@@ -214,7 +214,7 @@ struct LesImpl {
     double mean[kFeat + 1] = {0}, stdv[kFeat + 1] = {1, 1, 1, 1, 1};
     nse2d_cnn_t cnn = nullptr;
     int attach_mode = 1;   // 1: closure of every stage state; 2: of the step's first stage, held
-    long long hook_calls = 0;
+    const double* attached_w = nullptr;  // omega^n of the attached channel (its first stage state)
 };
 
 namespace {
@@ -262,19 +262,25 @@ Norm raw_norm() {
     return n;
 }
 
-// the closure hook of a coarse channel (nse2d_closure_fn)
-void closure_hook(void* ctx, const double* w, const double* psi, float* dst, hipStream_t s) {
-    LesImpl* L = static_cast<LesImpl*>(ctx);
-    (void)s;  // the channel and L share one stream (nse2d_les_attach)
-    // mode 2: the channel calls the hook before each of the three stages of a step; only the
-    // first call of a step evaluates, and the channel's buffer keeps that term for the others
-    if (L->attach_mode == 2 && (L->hook_calls++ % 3) != 0) return;
-    // the inputs straight into the network's packed input, the closure term straight into dst
+// The closure term of the coarse state (w, psi) into dst, on L's stream: the inputs go straight
+// into the network's packed input, the scaled term straight into dst.
+void closure_eval(LesImpl* L, const double* w, const double* psi, float* dst) {
     const Grid& g = L->c;
     features_kernel<true><<<dim3(ceil_div(g.d.Nx, 256), g.d.Ny), 256, 0, L->s>>>(
         w, psi, g.d.Nx, g.d.Ny, g.inv_dx, g.inv_deta, g.d.d_a_node, cnn_packed_input(L->cnn), L->norm);
     NSE2D_LAUNCH_CHECK();
     cnn_infer_packed_closure(L->cnn, (float)L->stdv[kFeat], (float)L->mean[kFeat], dst);
+}
+
+// The closure hook of a coarse channel (nse2d_closure_fn). The channel runs on L's stream
+// (checked by nse2d_les_attach). Mode 2: the channel calls the hook before each of the three
+// stages of a step with that stage's state; only the first stage's state (the channel's own
+// omega^n field) is evaluated, and the channel's buffer keeps that term for the other two.
+void closure_hook(void* ctx, const double* w, const double* psi, float* dst, hipStream_t s) {
+    LesImpl* L = static_cast<LesImpl*>(ctx);
+    (void)s;
+    if (L->attach_mode == 2 && w != L->attached_w) return;
+    closure_eval(L, w, psi, dst);
 }
 
 std::vector<double> read_partials(LesImpl* L, size_t n) {
@@ -508,13 +514,20 @@ int nse2d_les_eval(nse2d_les_t L, int first, int count, double* loss_cnn, double
     return les_eval(L->impl, first, count, loss_cnn, loss_zero);
 }
 int nse2d_les_closure(nse2d_les_t L, const double* d_omega_c, const double* d_psi_c, float* d_out) {
-    closure_hook(L->impl, d_omega_c, d_psi_c, d_out, L->impl->s);
+    closure_eval(L->impl, d_omega_c, d_psi_c, d_out);
     return 0;
 }
 int nse2d_les_attach(nse2d_les_t L, nse2d_channel_t coarse_ch, int on) {
     if (on < 0 || on > 2) { set_last_error("nse2d_les_attach: mode must be 0, 1 or 2"); return -1; }
+    if (!coarse_ch) { set_last_error("nse2d_les_attach: null channel"); return -1; }
+    if (on && nse2d_channel_stream(coarse_ch) != L->impl->s) {
+        // the stage kernel reads the closure buffer that the hook's inference writes: one stream
+        // orders them
+        set_last_error("nse2d_les_attach: the channel must run on the stream the LES was created with");
+        return -1;
+    }
     L->impl->attach_mode = on;
-    L->impl->hook_calls = 0;
+    L->impl->attached_w = nse2d_channel_field(coarse_ch, NSE2D_FIELD_OMEGA);
     return nse2d_channel_set_closure(coarse_ch, on ? closure_hook : nullptr, L->impl);
 }
 

@@ -55,6 +55,15 @@ void nse2d_memcpy_d2d(void* dst, const void* src, size_t bytes) {
     NSE2D_HIP_CHECK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
 }
 void nse2d_device_sync(void) { NSE2D_HIP_CHECK(hipDeviceSynchronize()); }
+hipStream_t nse2d_stream_create(void) {
+    hipStream_t s = nullptr;
+    NSE2D_HIP_CHECK(hipStreamCreate(&s));  // default priority
+    return s;
+}
+void nse2d_stream_destroy(hipStream_t s) {
+    if (s) NSE2D_HIP_CHECK(hipStreamDestroy(s));
+}
+void nse2d_stream_sync(hipStream_t s) { NSE2D_HIP_CHECK(hipStreamSynchronize(s)); }
 int nse2d_device_count(void) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) return 0;

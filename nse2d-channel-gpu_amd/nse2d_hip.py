@@ -99,6 +99,7 @@ SIGNATURES = {
     "nse2d_cnn_eval_loss": (_d, [_vp, _vp, _vp, _i]),
     # learned closure of a coarse channel (LES)
     "nse2d_channel_set_closure": (_i, [_vp, _vp, _vp]),
+    "nse2d_channel_stream": (_vp, [_vp]),
     "nse2d_les_create": (_vp, [_vp, _vp, _i, _i, ctypes.c_ulonglong, _stream]),
     "nse2d_les_destroy": (None, [_vp]),
     "nse2d_les_cnn": (_vp, [_vp]),
@@ -134,6 +135,9 @@ SIGNATURES = {
     "nse2d_memcpy_d2h": (None, [_vp, _vp, _sz]),
     "nse2d_memcpy_d2d": (None, [_vp, _vp, _sz]),
     "nse2d_device_sync": (None, []),
+    "nse2d_stream_create": (_vp, []),
+    "nse2d_stream_destroy": (None, [_vp]),
+    "nse2d_stream_sync": (None, [_vp]),
     "nse2d_device_count": (_i, []),
     "nse2d_last_error": (ctypes.c_char_p, []),
     "nse2d_version": (ctypes.c_char_p, []),
@@ -151,7 +155,7 @@ class ChannelDesc(ctypes.Structure):
                 ("nforce", _i), ("psi_bot", _d), ("psi_top", _d), ("mms", _i), ("mms_A0", _d),
                 ("mms_eps", _d), ("mms_Om", _d), ("mms_kx", _i), ("cfl", _d), ("cvisc", _d),
                 ("dt_max", _d), ("dy_min", _d), ("d_a_node", _vp), ("d_a_edge", _vp), ("d_y_node", _vp),
-                ("d_dy_edge", _vp), ("d_w_node", _vp)]
+                ("d_dy_edge", _vp), ("d_w_node", _vp), ("wall_type", _i)]
 
 
 _LIB = None
@@ -243,6 +247,11 @@ class DeviceArray:
         out = np.empty(self.shape, self.dtype)
         lib().nse2d_memcpy_d2h(out.ctypes.data, self.ptr, self.nbytes)
         return out
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes
+        lib().nse2d_memcpy_h2d(self.ptr, a.ctypes.data, self.nbytes)
 
     def free(self):
         if self.ptr:

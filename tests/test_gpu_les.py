@@ -231,6 +231,82 @@ def test_closure_term_matches_the_cnn_oracle():
     assert e <= 5e-3
 
 
+def test_once_per_step_closure_with_direct_calls_between():
+    """Mode 2 (closure of each step's first stage state, held over the three stages) with direct
+    nse2d_les_closure calls on other states between the steps: every direct call writes its
+    output (the buffer is filled with NaN before each), repeated calls agree bit for bit, and each
+    attached step equals the oracle step carrying the closure of that step's own first stage state
+    as a fixed source (the stage-hook bars)."""
+    c = cfg(64, 33)
+    F = lx.Channel(cfg(128, 65), ic=False)
+    C = lx.Channel(c, ic=False)
+    les = lx.Les(F, C, 1, 1, seed=3)
+    g = Grid(64, 33, 2.0)
+    mean = np.array([0.5, 1.0, 0.7, 0.0, 0.1])
+    std = np.array([3.0, 0.6, 0.5, 0.2, 2.0])
+    assert nh.lib().nse2d_les_set_scales(les.h, mean.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                         std.ctypes.data_as(ctypes.POINTER(ctypes.c_double))) == 0
+    w0, _ = fine_state(g, 2.0, 6)
+    C.set_omega(w0)
+    les.attach(C, mode=2)
+    other_w, other_psi = fine_state(g, 2.0, 9)
+    d_ow, d_opsi = nh.DeviceBuffer.from_array(other_w), nh.DeviceBuffer.from_array(other_psi)
+    d_out = nh.DeviceArray(shape=(33, 64), dtype=np.float32)
+    nan = np.full((33, 64), np.nan, np.float32)
+
+    def direct(wp, pp):
+        d_out.upload(nan)
+        assert nh.lib().nse2d_les_closure(les.h, wp, pp, d_out.ptr) == 0
+        out = d_out.download()
+        assert np.all(np.isfinite(out))
+        return out
+
+    dt, t = 2e-4, 0.0
+    try:
+        for k in range(3):
+            w_n, psi_n = C.omega(), nh.download_ptr(C.field(nh.FIELD_PSI), g.Nx * g.Ny, (g.Ny, g.Nx))
+            T = direct(C.field(nh.FIELD_OMEGA), C.field(nh.FIELD_PSI))
+            X = [direct(d_ow.ptr, d_opsi.ptr) for _ in range(k + 1)]
+            assert all(np.array_equal(X[0], x) for x in X)
+            nh.lib().nse2d_channel_step(C.ch, t, dt, 0)
+            sync()
+            o = SourceStepper(g, C.nu, 0.05, 12, 1e-3, 2.0, source=T.astype(np.float64))
+            psi_o, w_o = o.step(psi_n.copy(), w_n.copy(), t, dt)
+            e = step_errors(g, C.omega(), nh.download_ptr(C.field(nh.FIELD_PSI), g.Nx * g.Ny, (g.Ny, g.Nx)),
+                            w_o, psi_o, 2.0)
+            print(f"[les] mode-2 step {k} with {k + 1} direct calls before it: psi {e[0]:.2e}, interior "
+                  f"{e[1]:.2e}, wall vs own closure {e[2]:.2e}", flush=True)
+            assert e[0] <= 2e-13 and e[1] <= 2e-13 and e[2] <= 1e-13
+            t += dt
+    finally:
+        les.attach(C, mode=0)
+        les.destroy()
+        F.destroy()
+        C.destroy()
+
+
+def test_attach_rejects_a_channel_on_another_stream():
+    """The stage kernel reads the closure buffer that the hook's inference writes on the LES's
+    stream, so the channel must run on that stream."""
+    c = cfg(64, 33)
+    F = lx.Channel(cfg(128, 65), ic=False)
+    C = lx.Channel(c, ic=False)
+    les = lx.Les(F, C, 1, 1, seed=3)
+    s = nh.lib().nse2d_stream_create()
+    C2 = lx.Channel(c, stream=s, ic=False)
+    try:
+        assert nh.lib().nse2d_les_attach(les.h, C2.ch, 1) == -1
+        assert "stream" in nh.last_error()
+        assert nh.lib().nse2d_les_attach(les.h, C.ch, 1) == 0
+        assert nh.lib().nse2d_les_attach(les.h, C.ch, 0) == 0
+    finally:
+        les.destroy()
+        C2.destroy()
+        C.destroy()
+        F.destroy()
+        nh.lib().nse2d_stream_destroy(s)
+
+
 def test_training_on_filtered_dns_beats_the_zero_closure():
     """A small DNS (512x257, Re_b = 5000), 64 pairs filtered 8x8 to 64x33, 300 Adam steps: the
     trained closure must explain most of the hold-out target (normalised MSE below half that of

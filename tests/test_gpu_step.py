@@ -89,7 +89,7 @@ class OracleStepper:
         return psi, w
 
 
-def make_channel(g, nu, F0, nforce, drag, psi_top, mms=None, cfl=0.5, cvisc=0.5, dt_max=1e-2):
+def make_channel(g, nu, F0, nforce, drag, psi_top, mms=None, cfl=0.5, cvisc=0.5, dt_max=1e-2, wall_type=0):
     d = nh.ChannelDesc()
     d.Nx, d.Ny, d.Lx, d.h = g.Nx, g.Ny, g.Lx, g.h
     d.nu, d.F0, d.lin_drag, d.nforce = nu, F0, drag, nforce
@@ -100,6 +100,7 @@ def make_channel(g, nu, F0, nforce, drag, psi_top, mms=None, cfl=0.5, cvisc=0.5,
     d.dy_min = float(np.min(np.abs(g.m.dy_edge)))
     d.d_a_node, d.d_a_edge, d.d_y_node = g.dp("a_node"), g.dp("a_edge"), g.dp("y")
     d.d_dy_edge, d.d_w_node = g.dp("dy_edge"), g.dp("w_node")
+    d.wall_type = wall_type
     ch = nh.lib().nse2d_channel_create(ctypes.byref(d), None)
     assert ch, nh.last_error()
     return ch
@@ -165,8 +166,7 @@ def stagewise_errors(Nx, Ny, mms, beta=2.0, nu=1e-3, w0=None, free_slip=0):
     solver = GpuPoisson(g)
     o = OracleStepper(g, nu, F0, nf, drag, 2.0, mms=mms, solver=solver)
     o.fs = free_slip
-    ch = make_channel(g, nu, F0, nf, drag, 2.0, mms=mms)
-    nh.lib().nse2d_set_wall_type(free_slip)
+    ch = make_channel(g, nu, F0, nf, drag, 2.0, mms=mms, wall_type=2 if free_slip else 1)
     nh.upload_ptr(nh.lib().nse2d_channel_field(ch, nh.FIELD_OMEGA), w0)
     nh.lib().nse2d_channel_init(ch)
     t, dt = 0.3, 1e-3
@@ -179,7 +179,6 @@ def stagewise_errors(Nx, Ny, mms, beta=2.0, nu=1e-3, w0=None, free_slip=0):
     W1, W2, W3 = (field(ch, k, g) for k in (nh.FIELD_W1, nh.FIELD_W2, nh.FIELD_OMEGA))
     psi_g, rhs_g = field(ch, nh.FIELD_PSI, g), field(ch, nh.FIELD_RHS, g)
     nh.lib().nse2d_channel_destroy(ch)
-    nh.lib().nse2d_set_wall_type(0)
     L, N = oracle.lib(), w_n.size
     errs = {}
 
@@ -231,6 +230,59 @@ def test_step_stagewise(Nx, Ny, mms, fs):
     errs = stagewise_errors(Nx, Ny, mms, free_slip=fs)
     print(f"[stagewise] {Nx}x{Ny}: " + _fmt(errs), flush=True)
     _check_stagewise(errs)
+
+
+def test_wall_type_is_per_channel():
+    """A no-slip and a free-slip channel (descriptor wall_type 1 and 2) stepped in turn, with the
+    library-wide default set to the other closure in between: each follows the oracle step with its
+    own closure (ops.cu:108-156), so the choice lives in the channel, not in process state."""
+    Nx, Ny = 512, 33
+    g = Grid(Nx, Ny, 2.0)
+    nu, F0, nf, drag, dt = 1e-3, 0.05, 12, 1e-3, 1e-3
+    w0 = channel_ic(Nx, Ny)
+    chans, orcs = [], []
+    for fs in (0, 1):
+        ch = make_channel(g, nu, F0, nf, drag, 2.0, wall_type=2 if fs else 1)
+        nh.upload_ptr(nh.lib().nse2d_channel_field(ch, nh.FIELD_OMEGA), w0)
+        nh.lib().nse2d_set_wall_type(1 - fs)
+        nh.lib().nse2d_channel_init(ch)
+        o = OracleStepper(g, nu, F0, nf, drag, 2.0, free_slip=fs)
+        w_o = w0.copy()
+        chans.append(ch)
+        orcs.append([o, o.solve(w_o), w_o])
+    try:
+        for k in range(3):
+            for fs in (0, 1):
+                nh.lib().nse2d_set_wall_type(1 - fs)
+                nh.lib().nse2d_channel_step(chans[fs], k * dt, dt, 0)
+                o, psi_o, w_o = orcs[fs]
+                orcs[fs][1], orcs[fs][2] = o.step(psi_o, w_o, k * dt, dt)
+        sync()
+        for fs in (0, 1):
+            w_g, psi_g = field(chans[fs], nh.FIELD_OMEGA, g), field(chans[fs], nh.FIELD_PSI, g)
+            e_psi, e_int, e_cl, _ = step_errors(g, w_g, psi_g, orcs[fs][2], orcs[fs][1], 2.0, free_slip=fs)
+            print(f"[wall type per channel] free_slip={fs}: psi {e_psi:.2e}, interior {e_int:.2e}, "
+                  f"walls vs own closure {e_cl:.2e}", flush=True)
+            assert e_psi <= 1e-12 and e_int <= 1e-11 and e_cl <= 1e-13
+            if fs:
+                assert np.all(w_g[[0, -1]] == 0.0)
+            else:
+                assert np.min(np.abs(w_g[[0, -1]])) > 0
+    finally:
+        nh.lib().nse2d_set_wall_type(0)
+        for ch in chans:
+            nh.lib().nse2d_channel_destroy(ch)
+
+
+def test_invalid_wall_type_is_rejected():
+    g = Grid(64, 17, 2.0)
+    d = nh.ChannelDesc()
+    d.Nx, d.Ny, d.Lx, d.h, d.nu = g.Nx, g.Ny, g.Lx, g.h, 1e-3
+    d.d_a_node, d.d_a_edge, d.d_y_node = g.dp("a_node"), g.dp("a_edge"), g.dp("y")
+    d.d_dy_edge, d.d_w_node = g.dp("dy_edge"), g.dp("w_node")
+    d.wall_type = 3
+    assert not nh.lib().nse2d_channel_create(ctypes.byref(d), None)
+    assert "wall_type" in nh.last_error()
 
 
 STAGE_TILES = (["0"] + [f"{ry}x{sy}x{pf}" for ry, sy in ((8, 4), (16, 4), (16, 2), (32, 1)) for pf in (1, 2, 3)]
