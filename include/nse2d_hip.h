@@ -215,6 +215,17 @@ void launch_omega_rhs_balance_mapped(const double* d_omega, const double* d_rhs,
 void launch_max_uv_from_psi_mapped(const double* d_psi, int Nx, int Ny, double inv_dx,
                                    double inv_deta, const double* d_a_node, double* d_umax,
                                    double* d_vmax, double* d_vmaxm, hipStream_t stream);
+/* No counterpart in the reference. launch_max_uv_from_psi_mapped for callers that cannot
+ * allocate device memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc. Same kernels and results; the row partials live in a fixed device array of
+ * 3 x NSE2D_NOALLOC_MAX_ROWS doubles (393 KB) that each GPU holds until the process ends,
+ * instead of the library's grow-on-demand scratch. Returns 0, or -1 with nse2d_last_error() when
+ * Ny > NSE2D_NOALLOC_MAX_ROWS (nothing is launched then). All calls on one GPU share that array:
+ * calls that overlap on one GPU (concurrent streams or host threads) race. */
+#define NSE2D_NOALLOC_MAX_ROWS 16385
+int launch_max_uv_from_psi_mapped_noalloc(const double* d_psi, int Nx, int Ny, double inv_dx,
+                                          double inv_deta, const double* d_a_node, double* d_umax,
+                                          double* d_vmax, double* d_vmaxm, hipStream_t stream);
 
 /* ------------------------------------------------------------------------------------------
  * 5. Whole-step entry points: the SSPRK(3,3) step of main.cu:482-538 with each stage's

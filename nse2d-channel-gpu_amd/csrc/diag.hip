@@ -7,7 +7,17 @@
 #include "nse2d_ops.hpp"
 #include "../../include/nse2d_hip.h"
 
+#include <mutex>
+#include <string>
+
+// Row partials of launch_max_uv_from_psi_mapped_noalloc: a fixed array that each GPU holds from
+// the library's load until the process ends (3 x NSE2D_NOALLOC_MAX_ROWS doubles, 393 KB). Host
+// code never uses this name as an address: the device address comes from hipGetSymbolAddress.
+static __device__ __align__(256) double scratch[3 * NSE2D_NOALLOC_MAX_ROWS];
+
 namespace nse2d {
+
+void set_last_error(const std::string& msg);
 
 namespace {
 
@@ -297,18 +307,58 @@ void launch_max_uv_group(const MaxUVJob* d_jobs, int n, int total_rows, hipStrea
     NSE2D_LAUNCH_CHECK();
 }
 
-void launch_max_uv(const double* psi, int Nx, int Ny, double inv_dx, double inv_deta, const double* a_node,
-                   double* umax, double* vmax, double* vmaxm, hipStream_t s) {
-    if (Nx < 2 || Ny < 3) {  // diag.cu:826-832
-        zero3_kernel<<<1, 64, 0, s>>>(umax, vmax, vmaxm);
-        NSE2D_LAUNCH_CHECK();
-        return;
-    }
-    double* partial = scratch_doubles((size_t)3 * Ny);
+namespace {
+
+// The two launches of the CFL maxima with the row partials in `partial` (3 x Ny doubles; every
+// row writes its three entries, so nothing needs clearing between calls).
+void launch_max_uv_into(const double* psi, int Nx, int Ny, double inv_dx, double inv_deta, const double* a_node,
+                        double* umax, double* vmax, double* vmaxm, double* partial, hipStream_t s) {
     row_max_uv_kernel<<<Ny, RB, 0, s>>>(psi, Nx, Ny, inv_dx, inv_deta, a_node, partial);
     NSE2D_LAUNCH_CHECK();
     final_kernel<3, true><<<1, RB, 0, s>>>(partial, Ny, Outs<3>{{umax, vmax, vmaxm}}, 1.0);
     NSE2D_LAUNCH_CHECK();
+}
+
+void launch_zero3(double* umax, double* vmax, double* vmaxm, hipStream_t s) {  // diag.cu:826-832
+    zero3_kernel<<<1, 64, 0, s>>>(umax, vmax, vmaxm);
+    NSE2D_LAUNCH_CHECK();
+}
+
+// Device address of `scratch` on the current device, looked up once per device with that device
+// current (each GPU holds its own copy at its own address). nullptr with the error set on failure.
+constexpr int kNoallocDevices = 64;
+std::mutex g_noalloc_mu;
+double* g_noalloc_ptr[kNoallocDevices] = {};
+
+double* noalloc_scratch() {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kNoallocDevices) {
+        set_last_error("launch_max_uv_from_psi_mapped_noalloc: no usable current device");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t rc = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (rc != hipSuccess || !p) {
+            set_last_error(std::string("launch_max_uv_from_psi_mapped_noalloc: hipGetSymbolAddress failed: ") +
+                           hipGetErrorString(rc));
+            return nullptr;
+        }
+        g_noalloc_ptr[dev] = static_cast<double*>(p);
+    }
+    return g_noalloc_ptr[dev];
+}
+
+}  // namespace
+
+void launch_max_uv(const double* psi, int Nx, int Ny, double inv_dx, double inv_deta, const double* a_node,
+                   double* umax, double* vmax, double* vmaxm, hipStream_t s) {
+    if (Nx < 2 || Ny < 3) {
+        launch_zero3(umax, vmax, vmaxm, s);
+        return;
+    }
+    launch_max_uv_into(psi, Nx, Ny, inv_dx, inv_deta, a_node, umax, vmax, vmaxm, scratch_doubles((size_t)3 * Ny), s);
 }
 
 }  // namespace nse2d
@@ -419,6 +469,24 @@ void launch_max_uv_from_psi_mapped(const double* d_psi, int Nx, int Ny, double i
                                    const double* d_a_node, double* d_umax, double* d_vmax, double* d_vmaxm,
                                    hipStream_t stream) {
     launch_max_uv(d_psi, Nx, Ny, inv_dx, inv_deta, d_a_node, d_umax, d_vmax, d_vmaxm, stream);
+}
+
+int launch_max_uv_from_psi_mapped_noalloc(const double* d_psi, int Nx, int Ny, double inv_dx, double inv_deta,
+                                          const double* d_a_node, double* d_umax, double* d_vmax,
+                                          double* d_vmaxm, hipStream_t stream) {
+    if (Ny > NSE2D_NOALLOC_MAX_ROWS) {
+        set_last_error("launch_max_uv_from_psi_mapped_noalloc: Ny = " + std::to_string(Ny) + " exceeds the " +
+                       std::to_string(NSE2D_NOALLOC_MAX_ROWS) + " rows of its fixed scratch");
+        return -1;
+    }
+    if (Nx < 2 || Ny < 3) {
+        launch_zero3(d_umax, d_vmax, d_vmaxm, stream);
+        return 0;
+    }
+    double* partial = noalloc_scratch();
+    if (!partial) return -1;
+    launch_max_uv_into(d_psi, Nx, Ny, inv_dx, inv_deta, d_a_node, d_umax, d_vmax, d_vmaxm, partial, stream);
+    return 0;
 }
 
 }  // extern "C"
