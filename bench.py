@@ -92,6 +92,18 @@ def read_profile(ch):
     return {nh.PROF_CLASSES[k]: (ms[k], n[k]) for k in range(6)}
 
 
+def cpu_model():
+    """The host CPU's model name (the lscpu "Model name" field, read from /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(c, budget_s):
     """The oracle (FP64 OpenMP restatement) on this host: one warm step, then as many steps as fit
     in about budget_s seconds (at least 1)."""
@@ -108,6 +120,7 @@ def cpu_baseline(c, budget_s):
     el = L.orc_time_steps(len(args), cargv(args), 1, steps)
     pts = c["Nx"] * c["Ny"] * steps
     return {"value": pts / el, "unit": UNIT, "cores": int(L.orc_num_threads()), "kind": "port",
+            "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
             "sample": f"{steps} SSPRK(3,3) steps (adaptive dt) of the {c['Nx']}x{c['Ny']} workload after one "
                       f"warm step, FP64 OpenMP oracle, {el:.2f} s"}
 
@@ -383,6 +396,9 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "stage+fft_forward" if (fused and dom == "stage") else dom,
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": "profiles/traffic.json: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of "
+                                       "this workload, scaled by a 256 MiB copy (not measured in this run)"
+                                       if traffic is not None else None,
                      "avg_launch_us": avg_s * 1e6, "alg_bytes_per_launch": bytes_per_launch,
                      "timed_launches": int(dom_n), "timed_every_k_steps": a.time_every},
         # 240 B/point: the unfused step (three stages, three Poisson solves, the CFL read); the

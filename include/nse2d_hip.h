@@ -63,38 +63,38 @@ void testapi_arakawa_J_mapped_split(const double* d_psi, const double* d_omg, do
                                     hipStream_t s);
 
 /* ------------------------------------------------------------------------------------------
- * 2. Operator launchers (ops.cuh:268-337), same names and parameter order, C linkage.
+ * 2. Operator launchers (ops.cuh:12-81), same names and parameter order, C linkage.
  * ------------------------------------------------------------------------------------------ */
-/* ops.cuh:268-269, ops.cu:64-70 */
+/* ops.cuh:12-13, ops.cu:64-70 */
 void launch_uv_from_psi_mapped(const double* psi, double* u, double* v, int Nx, int Ny,
                                double inv_dx, double inv_deta, const double* d_a_node,
                                hipStream_t s);
-/* ops.cuh:274-276, ops.cu:94-101; writes interior rows only */
+/* ops.cuh:18-20, ops.cu:94-101; writes interior rows only */
 void launch_omega_from_psi_interior_mapped(double* w, const double* psi, int Nx, int Ny,
                                            double inv_dx2, double inv_deta2,
                                            const double* d_a_node, const double* d_a_edge,
                                            hipStream_t s);
-/* ops.cuh:279-280, ops.cu:270-276; wall type from nse2d_set_wall_type() */
+/* ops.cuh:23-24, ops.cu:270-276; wall type from nse2d_set_wall_type() */
 void launch_thom_wall_vorticity_mapped(double* w, const double* psi, int Nx, int Ny,
                                        const double* d_dy_edge, double psi_bot, double psi_top,
                                        hipStream_t s);
-/* ops.cuh:287-288, ops.cu:328-334 */
+/* ops.cuh:31-32, ops.cu:328-334 */
 void launch_arakawa_J_mapped(const double* psi, const double* omg, double* J, int Nx, int Ny,
                              double inv_dx, double inv_deta, const double* d_a_node,
                              hipStream_t s);
-/* ops.cuh:312-316, ops.cu:501-512 */
+/* ops.cuh:56-60, ops.cu:501-512 */
 void launch_laplacian_fd_mapped(const double* d_q, double* d_out, int Nx, int Ny,
                                 double inv_dx2, double inv_deta2, const double* d_a_node,
                                 const double* d_a_edge, hipStream_t s);
-/* ops.cuh:322-325, ops.cu:536-544 */
+/* ops.cuh:66-69, ops.cu:536-544 */
 void launch_rhs_mapped(const double* J, const double* lap, const double* omg, double* rhs,
                        int Nx, int Ny, double nu, double F0, int nforce, double h,
                        double lin_drag, const double* d_y_node, hipStream_t s);
-/* ops.cuh:326-331, ops.cu:628-641 */
+/* ops.cuh:70-75, ops.cu:628-641 */
 void launch_mms_add_forcing(double* d_rhs, int Nx, int Ny, double Lx, double h, double t,
                             double nu, double r, double A0, double eps, double Om, int kx_mode,
                             const double* d_y_node, hipStream_t s);
-/* ops.cuh:335-337, ops.cu:656-669; launch_ssprk3 allows wnp1 == w (main.cu:530) */
+/* ops.cuh:79-81, ops.cu:656-669; launch_ssprk3 allows wnp1 == w (main.cu:530) */
 void launch_axpy(double* y, const double* x, const double* r, double dt, size_t N,
                  hipStream_t s);
 void launch_ssprk2(double* w2, const double* w, const double* w1, const double* rhs, double dt,
