@@ -6,6 +6,7 @@
 #include "nse2d_oracle.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <complex>
@@ -366,6 +367,12 @@ extern "C" void orc_ssprk3(double* wnp1, const double* w, const double* w2, cons
 // Poisson: poisson.cu + poisson.cuh:83-118, with the project's own FFT in place of cuFFT.
 // =============================================================================================
 namespace {
+// Evaluation order of the solve (bit flags): 1 = power-of-two FFTs by decimation in frequency
+// instead of in time, 2 = y elimination from the top wall down instead of from the bottom up. A
+// measurement tool with no reference counterpart: two FP64 evaluations of the same solver that
+// round differently show how closely any two FP64 implementations can agree.
+std::atomic<int> g_eval_order{0};
+
 struct FFTPlan {
     int N = 0;
     bool pow2 = false;
@@ -404,6 +411,25 @@ struct FFTPlan {
             }
             return;
         }
+        if (g_eval_order.load(std::memory_order_relaxed) & 1) {
+            // decimation in frequency (Gentleman-Sande): butterflies from the full length down,
+            // the bit-reversal permutation last. The same transform as the decimation-in-time
+            // loop below, with every value rounded in another order.
+            for (int len = N; len >= 2; len >>= 1) {
+                const int half = len >> 1, step = N / len;
+                for (int s = 0; s < N; s += len) {
+                    for (int k = 0; k < half; ++k) {
+                        const std::complex<double> w = inverse ? std::conj(tw[(size_t)k * step]) : tw[(size_t)k * step];
+                        const std::complex<double> u = a[s + k], v = a[s + k + half];
+                        a[s + k] = u + v;
+                        a[s + k + half] = (u - v) * w;
+                    }
+                }
+            }
+            for (int i = 0; i < N; ++i)
+                if (i < rev[i]) std::swap(a[i], a[rev[i]]);
+            return;
+        }
         for (int i = 0; i < N; ++i)
             if (i < rev[i]) std::swap(a[i], a[rev[i]]);
         for (int len = 2; len <= N; len <<= 1) {
@@ -421,6 +447,9 @@ struct FFTPlan {
     }
 };
 }  // namespace
+
+extern "C" void orc_set_eval_order(int flags) { g_eval_order.store(flags & 3); }
+extern "C" int orc_get_eval_order(void) { return g_eval_order.load(); }
 
 extern "C" void orc_fft_z2z(double* data, int N, int howmany, int inverse) {
     FFTPlan plan(N);
@@ -489,6 +518,45 @@ extern "C" void orc_poisson_tridiag_mapped(double* hat, const double* cp, const 
     typedef std::complex<double> C;
     C* H = reinterpret_cast<C*>(hat);
     const int Ni = Ny - 2;
+    if (g_eval_order.load() & 2) {
+        // The same systems eliminated from the top interior row down (x_r = e_r - q_r x_{r-1},
+        // q_r = a_r / den_r, e_r = (f_r - c_r e_{r+1}) / den_r, den_r = b_r - c_r q_{r+1}), then
+        // substituted upward: exact in real arithmetic, rounded in another order.
+#pragma omp parallel
+        {
+            std::vector<double> q(Ni);
+            std::vector<C> e(Ni);
+#pragma omp for schedule(static)
+            for (int k = 0; k < Nx; ++k) {
+                C* ph = H + k;
+                double qn = 0.0;
+                C en(0.0, 0.0);
+                for (int r = Ni - 1; r >= 0; --r) {
+                    const int j = r + 1;
+                    const double a_sub = a_node[j] * a_edge[j - 1] * inv_deta2;
+                    const double c_sup = a_node[j] * a_edge[j] * inv_deta2;
+                    const double b = -(a_sub + c_sup) - kx2[k];
+                    C f = -ph[(size_t)Nx * j];
+                    if (k == 0 && r == 0) f = C(f.real() - a_sub * psi_bot * (double)Nx, f.imag());
+                    if (k == 0 && r == Ni - 1) f = C(f.real() - c_sup * psi_top * (double)Nx, f.imag());
+                    const double den = b - c_sup * qn;
+                    q[r] = a_sub / den;
+                    e[r] = C((f.real() - c_sup * en.real()) / den, (f.imag() - c_sup * en.imag()) / den);
+                    qn = q[r];
+                    en = e[r];
+                }
+                C xp = e[0];
+                ph[(size_t)Nx] = xp;
+                for (int r = 1; r < Ni; ++r) {
+                    xp = C(e[r].real() - q[r] * xp.real(), e[r].imag() - q[r] * xp.imag());
+                    ph[(size_t)Nx * (r + 1)] = xp;
+                }
+                ph[0] = (k == 0 ? C((double)Nx * psi_bot, 0.0) : C(0.0, 0.0));
+                ph[(size_t)Nx * (Ny - 1)] = (k == 0 ? C((double)Nx * psi_top, 0.0) : C(0.0, 0.0));
+            }
+        }
+        return;
+    }
 #pragma omp parallel for schedule(static)
     for (int k = 0; k < Nx; ++k) {
         C* ph = H + k;  // stride Nx

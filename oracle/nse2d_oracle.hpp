@@ -68,6 +68,13 @@ void orc_ssprk3(double* wnp1, const double* w, const double* w2, const double* r
 // convention used at poisson.cuh:96,110). Radix-2 for powers of two, direct O(N^2) otherwise.
 void orc_fft_z2z(double* data, int N, int howmany, int inverse);
 void orc_dft_naive(const double* in, double* out, int N, int inverse);                        // textbook DFT
+// Evaluation order of every transform and y elimination of the oracle (orc_fft_z2z, the Poisson
+// solves, the driver), bit flags: 1 = FFTs by decimation in frequency instead of in time, 2 = the
+// y systems eliminated from the top wall down instead of from the bottom up. A measurement tool
+// with no reference counterpart: the same solver evaluated in a second FP64 order shows how
+// closely two FP64 implementations of the path can agree.
+void orc_set_eval_order(int flags);
+int orc_get_eval_order(void);
 void orc_fill_kx2(double* kx2, int Nx, double Lx);                                            // poisson.cu:16-22
 void orc_precompute_cp_mapped(double* cp, const double* a_node, const double* a_edge,
                             const double* kx2, int Nx, int Ny, double inv_deta2);             // poisson.cu:38-68

@@ -29,6 +29,8 @@ SIG = {
     "orc_ssprk3": (None, [_vp, _vp, _vp, _vp, _d, _sz]),
     "orc_fft_z2z": (None, [_vp, _i, _i, _i]),
     "orc_dft_naive": (None, [_vp, _vp, _i, _i]),
+    "orc_set_eval_order": (None, [_i]),
+    "orc_get_eval_order": (_i, []),
     "orc_fill_kx2": (None, [_vp, _i, _d]),
     "orc_precompute_cp_mapped": (None, [_vp, _vp, _vp, _vp, _i, _i, _d]),
     "orc_poisson_tridiag_mapped": (None, [_vp, _vp, _vp, _vp, _vp, _i, _i, _d, _d, _d]),

@@ -13,13 +13,20 @@ from oracle import P
 L = oracle.lib
 
 
+@pytest.mark.parametrize("order", [0, 1], ids=["dit", "dif"])
 @pytest.mark.parametrize("N", [2, 4, 8, 16, 64, 256, 1024, 12, 30])
-def test_fft_matches_numpy_and_naive_dft(N):
+def test_fft_matches_numpy_and_naive_dft(N, order):
+    """Both evaluation orders of the oracle's FFT (decimation in time, the default, and in
+    frequency, a measurement tool) against numpy and the O(N^2) DFT."""
     rng = np.random.default_rng(N)
     z = rng.normal(size=N) + 1j * rng.normal(size=N)
     for inv in (0, 1):
         buf = np.ascontiguousarray(np.stack([z.real, z.imag], axis=1).ravel())
-        L().orc_fft_z2z(P(buf), N, 1, inv)
+        L().orc_set_eval_order(order)
+        try:
+            L().orc_fft_z2z(P(buf), N, 1, inv)
+        finally:
+            L().orc_set_eval_order(0)
         got = buf[0::2] + 1j * buf[1::2]
         ref = np.fft.ifft(z) * N if inv else np.fft.fft(z)
         assert np.max(np.abs(got - ref)) <= 1e-13 * np.max(np.abs(ref))
@@ -28,6 +35,51 @@ def test_fft_matches_numpy_and_naive_dft(N):
         L().orc_dft_naive(P(src), P(out), N, inv)
         naive = out[0::2] + 1j * out[1::2]
         assert np.max(np.abs(naive - ref)) <= 1e-13 * np.max(np.abs(ref))
+
+
+def test_fft_orders_round_differently():
+    """The two orders are different FP64 evaluations of one transform (else they could not
+    measure how closely two FP64 solvers agree): equal to rounding, not bit for bit."""
+    N = 4096
+    z = np.random.default_rng(1).normal(size=2 * N)
+    a, b = z.copy(), z.copy()
+    L().orc_fft_z2z(P(a), N, 1, 0)
+    L().orc_set_eval_order(1)
+    try:
+        L().orc_fft_z2z(P(b), N, 1, 0)
+    finally:
+        L().orc_set_eval_order(0)
+    d = np.max(np.abs(a - b)) / np.max(np.abs(a))
+    assert 0 < d < 1e-14, d
+
+
+def test_reversed_elimination_matches_dense_solve_and_rounds_differently():
+    """The elimination from the top wall down (evaluation order flag 2, a measurement tool) solves
+    the same operator: against the dense solve on a small grid, and at the benchmark's 2049 rows
+    it differs from the reference's bottom-up elimination by rounding only (the FP64 spread of the
+    y solve)."""
+    Nx, Ny, Lx = 8, 9, 5.0
+    m = oracle.Metrics(Ny, 1.0, 1, 2.0)
+    w = np.random.default_rng(5).normal(size=(Ny, Nx))
+    a = oracle.poisson_solve(w, Lx, m, 0.0, 2.0)
+    L().orc_set_eval_order(2)
+    try:
+        b = oracle.poisson_solve(w, Lx, m, 0.0, 2.0)
+    finally:
+        L().orc_set_eval_order(0)
+    np.testing.assert_allclose(b, a, rtol=0, atol=1e-14 * np.max(np.abs(a)))
+    Nx, Ny = 16, 2049
+    m = oracle.Metrics(Ny, 1.0, 1, 2.0)
+    w = np.random.default_rng(6).normal(size=(Ny, Nx)) + 3 * m.y[:, None]
+    a = oracle.poisson_solve(w, 2 * math.pi, m, 0.0, 2.0)
+    L().orc_set_eval_order(3)
+    try:
+        b = oracle.poisson_solve(w, 2 * math.pi, m, 0.0, 2.0)
+    finally:
+        L().orc_set_eval_order(0)
+    d = np.max(np.abs(a - b)) / np.max(np.abs(a))
+    print(f"bottom-up vs top-down elimination at 2049 rows: {d:.2e}")
+    assert 0 < d < 2e-12
 
 
 def test_kx2_convention():

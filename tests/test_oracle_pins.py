@@ -456,6 +456,10 @@ PINNED_BY = {
     "orc_laplacian_fd_mapped": [("test_oracle_math.py", "test_uniform_laplacian_closed_forms")],
     "orc_fft_z2z": [("test_oracle_math.py", "test_fft_matches_numpy_and_naive_dft")],
     "orc_dft_naive": [("test_oracle_math.py", "test_fft_matches_numpy_and_naive_dft")],
+    "orc_set_eval_order": [("test_oracle_math.py", "test_fft_matches_numpy_and_naive_dft"),
+                           ("test_oracle_math.py", "test_fft_orders_round_differently"),
+                           ("test_oracle_math.py",
+                            "test_reversed_elimination_matches_dense_solve_and_rounds_differently")],
     "orc_fill_kx2": [("test_oracle_math.py", "test_kx2_convention")],
     "orc_precompute_cp_mapped": [("test_oracle_math.py", "test_thomas_rounding_floor_at_2049_rows")],
     "orc_poisson_tridiag_mapped": [("test_oracle_math.py", "test_thomas_rounding_floor_at_2049_rows")],
@@ -472,7 +476,7 @@ VIA_FIXTURE_DRIVER = [
     "orc_enstrophy_input_mapped_full", "orc_row_moments_from_psi_mapped", "orc_mean_wall_omega",
     "orc_mean_wall_omega_omegay_mapped", "orc_omega_rhs_balance_mapped", "orc_max_uv_from_psi_mapped",
 ]
-TOOLS = ["orc_time_steps", "orc_num_threads"]   # timing helpers of the CPU baseline, no arithmetic
+TOOLS = ["orc_time_steps", "orc_num_threads", "orc_get_eval_order"]   # timing helpers of the CPU baseline, no arithmetic
 
 
 def test_every_exported_oracle_function_is_pinned():
