@@ -164,3 +164,50 @@ def test_staged_convolution_matches_direct(H, W, B):
         assert np.array_equal(res[m][1], res[1][1]), m
     for m in (0, 1, 2):
         assert np.array_equal(res[m][2], res[m][0]), m
+
+
+def test_forward_and_gradients_at_the_les_workload_on_a_subsample(progress):
+    """The network at the learned-closure workload (257x512 images, batch 32): the GPU runs the
+    forward and the backward of the whole batch; the output gradient is nonzero only on three row
+    bands (the two walls and the middle) of two images, so the oracle needs only those images,
+    cropped to each band with 16 rows of margin (the network sees 6 rows up and down; activations
+    within 10 rows of a cut edge are exact): outputs on the bands and every parameter gradient
+    (the sum over the crops) against the bf16-emulating oracle, at the bars of the small-size
+    tests."""
+    H, W, B = 257, 512, 32
+    progress(f"[cnn] workload {B}x{H}x{W}, oracle on a subsample")
+    c, p = make(H, W, B, seed=21)
+    lay, n = co.layout()
+    rng = np.random.default_rng(257)
+    inp = rng.normal(size=(B, H, W, co.NIN)).astype(np.float32)
+    bands = [(0, 6), (124, 132), (251, 257)]
+    images = (0, 17)
+    gout = np.zeros((B, H, W), np.float32)
+    for i in images:
+        for b0, b1 in bands:
+            gout[i, b0:b1] = rng.normal(size=(b1 - b0, W))
+    d_in, d_out, d_g = nh.DeviceArray(inp), nh.DeviceArray(shape=(B, H, W)), nh.DeviceArray(gout)
+    assert nh.lib().nse2d_cnn_forward(c, d_in.ptr, d_out.ptr, B) == 0
+    out = d_out.download().astype(np.float64)
+    assert nh.lib().nse2d_cnn_backward(c, d_g.ptr, B) == 0
+    g = np.zeros(n, np.float32)
+    nh.lib().nse2d_cnn_get_grads(c, g.ctypes.data)
+    nh.lib().nse2d_cnn_destroy(c)
+    p64 = p.astype(np.float64)
+    gr = np.zeros(n)
+    got_o, ref_o = [], []
+    for i in images:
+        for b0, b1 in bands:
+            lo, hi = max(0, b0 - 16), min(H, b1 + 16)
+            ref, cache = co.forward(p64, inp[i:i + 1, lo:hi].astype(np.float64), emulate=True)
+            got_o.append(out[i, b0:b1])
+            ref_o.append(ref[0, b0 - lo:b1 - lo])
+            gr += co.backward(p64, cache, gout[i:i + 1, lo:hi].astype(np.float64), emulate=True)
+    e = nrel(np.concatenate(got_o), np.concatenate(ref_o))
+    progress(f"[cnn] workload forward on the bands vs bf16 oracle {e:.2e}")
+    assert e <= 5e-3
+    for l, (ow, cout, cin, ob) in enumerate(lay):
+        sl = slice(ow, ow + 9 * cout * cin)
+        ew, eb = nrel(g[sl], gr[sl]), nrel(g[ob:ob + cout], gr[ob:ob + cout])
+        progress(f"[cnn] workload layer {l}: dW {ew:.2e} db {eb:.2e}")
+        assert ew <= 2e-2 and eb <= 2e-2

@@ -3,6 +3,7 @@ channel advanced alone (bitwise, same kernel shapes), graph replays equal direct
 member against the oracle with the host dt rule, and the autotuner leaves the state untouched."""
 import ctypes
 import math
+import os
 
 import numpy as np
 import pytest
@@ -223,3 +224,93 @@ def test_persistent_stage_equals_one_tile_per_workgroup(tile, monkeypatch):
     for m in range(len(S.members)):
         assert np.array_equal(fa[m][0], fb[m][0]) and np.array_equal(fa[m][1], fb[m][1]), m
     assert np.array_equal(sa, sb)
+
+
+def config4_members(seed=20241203):
+    """The 64-member ensemble workload: 32 channels at 512x257, 24 at 1024x513 and 8 at
+    2048x1025, Re_b log-uniform in [2e3, 1e4] (seeded), each with its own IC seed (the members of
+    bench.py --workload ensemble)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for Nx, Ny, count in ((512, 257, 32), (1024, 513, 24), (2048, 1025, 8)):
+        for _ in range(count):
+            out.append((Nx, Ny, float(np.exp(rng.uniform(np.log(2e3), np.log(1e4)))), 1000 + len(out)))
+    return out
+
+
+def test_config4_sixty_four_members(progress):
+    """The full 64-member ensemble, three graph-replayed steps with autotuned shapes: every member
+    equals its single-channel run under the chosen shapes bit for bit, and the first member of each
+    size class follows the oracle stepping with the host dt rule (main.cu:451-458)."""
+    members = config4_members()
+    progress(f"[ensemble cfg4] {len(members)} members, 3 graph steps")
+    S = Setup(members)
+    e = S.ensemble()
+    buf = ctypes.create_string_buffer(512)
+    assert nh.lib().nse2d_ensemble_autotune(e, 1, buf, 512) == 0
+    report = buf.value.decode()
+    progress("[ensemble cfg4] autotune: " + report)
+    nh.lib().nse2d_ensemble_sync(e)
+    start = [S.fields(e, m) for m in range(len(members))]
+    nh.lib().nse2d_ensemble_step_auto(e, 3)
+    nh.lib().nse2d_ensemble_sync(e)
+    sc = S.scalars(e)
+    assert np.all(np.isfinite(sc[:, :2]))
+    parts = [x.strip() for x in report.split(";")]
+    tile = parts[0].split()[1]
+    cls = {}
+    for part in parts[1:]:
+        size, rest = part.split(":")
+        Nx, Ny = (int(v) for v in size.split("x"))
+        cls[(Nx, Ny)] = (int(rest.split("radix")[1].split(",")[0]), int(rest.split("KC")[1]))
+    env = {k: os.environ.get(k) for k in ("NSE2D_STAGE_TILE", "NSE2D_FFT_RADIX", "NSE2D_THOMAS_KC")}
+    try:
+        os.environ["NSE2D_STAGE_TILE"] = tile
+        for m, (Nx, Ny, Reb, _) in enumerate(members):
+            os.environ["NSE2D_FFT_RADIX"], os.environ["NSE2D_THOMAS_KC"] = (str(v) for v in cls[(Nx, Ny)])
+            w_s, psi_s, t_s, dt_s = S.single(m, 3, start=start[m])
+            w_e, psi_e = S.fields(e, m)
+            assert np.array_equal(w_e, w_s) and np.array_equal(psi_e, psi_s), (m, np.max(np.abs(w_e - w_s)))
+            assert sc[m, 1] == dt_s, (m, sc[m, 1], dt_s)
+    finally:
+        for k, v in env.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    progress("[ensemble cfg4] 64 members equal their single-channel runs bit for bit")
+    for m in (0, 32, 56):
+        Nx, Ny, Reb, _ = members[m]
+        g = S.grids[(Nx, Ny)]
+        nu, cfl, cvisc, dt_max = 2.0 / Reb, 0.5, 0.5, 1e-2
+        dy_min = float(np.min(np.abs(g.m.dy_edge)))
+        res = []
+        for order in (0, 3):  # the oracle, and the oracle in the second FP64 order (the bar)
+            oracle.lib().orc_set_eval_order(order)
+            try:
+                o = OracleStepper(g, nu, 0.05, 12, 1e-3, 2.0)
+                w_o = S.ics[m].copy()
+                psi_o = o.solve(w_o)
+                t = 0.0
+                for k in range(3):
+                    mx = np.zeros(3)
+                    oracle.lib().orc_max_uv_from_psi_mapped(P(psi_o), Nx, Ny, g.inv_dx, g.inv_deta, P(g.m.a_node),
+                                                            P(mx))
+                    dt_x = cfl * (g.dx / mx[0]) if mx[0] > 0 else 1e30
+                    dt_y = cfl * ((2.0 / (Ny - 1)) / mx[2]) if mx[2] > 0 else 1e30
+                    dt = min(dt_max, min(min(dt_x, dt_y), cvisc * 0.5 / (nu * (g.inv_dx2 + 1.0 / dy_min ** 2))))
+                    psi_o, w_o = o.step(psi_o, w_o, t, dt)
+                    t += dt
+            finally:
+                oracle.lib().orc_set_eval_order(0)
+            res.append((w_o, psi_o, t))
+        (w_o, psi_o, t), (w_q, psi_q, _) = res
+        w_e, psi_e = S.fields(e, m)
+        ew, ei, ep = rel_err(w_e, w_o), rel_err(w_e[1:-1], w_o[1:-1]), rel_err(psi_e, psi_o)
+        fw, fi = rel_err(w_q, w_o), rel_err(w_q[1:-1], w_o[1:-1])
+        progress(f"[ensemble cfg4] member {m} ({Nx}x{Ny}, Re_b {Reb:.0f}) vs oracle after 3 steps: omega {ew:.2e} "
+                 f"(interior {ei:.2e}) psi {ep:.2e}; reordered oracle omega {fw:.2e} (interior {fi:.2e})")
+        assert abs(sc[m, 0] - t) <= 1e-14 * t
+        assert ep <= 1e-10
+        assert ew <= max(1e-10, 10.0 * fw) and ei <= max(1e-10, 10.0 * fi)
+    nh.lib().nse2d_ensemble_destroy(e)

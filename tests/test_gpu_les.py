@@ -80,8 +80,13 @@ def test_filter_matches_oracle(fine, coarse):
     assert e <= 1e-13
 
 
-@pytest.mark.parametrize("fine,coarse", [((256, 129), (64, 33)), ((512, 257), (64, 33))])
+@pytest.mark.parametrize("fine,coarse", [((256, 129), (64, 33)), ((512, 257), (64, 33)),
+                                         ((4096, 2049), (512, 257))],
+                         ids=["256x129-64x33", "512x257-64x33", "4096x2049-512x257"])
 def test_pair_matches_oracle(fine, coarse):
+    """The training pair (coarse state with Thom walls, closure target F(R_f) - R_c) against the FP64
+    oracle's composition, up to the learned-closure workload (4096x2049 DNS filtered 8x8 to the
+    512x257 LES grid)."""
     print(f"[les] pair {fine} -> {coarse}", flush=True)
     cf, cc = cfg(*fine), cfg(*coarse)
     F = lx.Channel(cf, ic=False)
@@ -107,8 +112,11 @@ def test_pair_matches_oracle(fine, coarse):
     tgt_o = np.zeros_like(frf)
     tgt_o[1:-1] = frf[1:-1] - rc[1:-1]
     e_w, e_psi = rel_err(wc[1:-1], wc_o[1:-1]), rel_err(psic, psic_o)
-    # the target is a difference of two RHS of similar size: its bar is relative to them
-    e_t = float(np.max(np.abs(tgt - tgt_o)) / np.max(np.abs(frf[1:-1])))
+    # the target is a difference of two RHS: its bar is relative to the larger of them (with a
+    # random fine state the coarse grid's own Thom walls make R_c near the walls far larger than
+    # the filtered F(R_f))
+    scale = max(np.max(np.abs(frf[1:-1])), np.max(np.abs(rc[1:-1])))
+    e_t = float(np.max(np.abs(tgt - tgt_o)) / scale)
     print(f"[les] pair: omega_c {e_w:.2e}, psi_c {e_psi:.2e}, target {e_t:.2e} "
           f"(|target| / |F(R_f)| = {np.max(np.abs(tgt_o)) / np.max(np.abs(frf[1:-1])):.2e})", flush=True)
     les.destroy()
@@ -231,17 +239,21 @@ def test_closure_term_matches_the_cnn_oracle():
     assert e <= 5e-3
 
 
-def test_once_per_step_closure_with_direct_calls_between():
+@pytest.mark.parametrize("fine,coarse", [((128, 65), (64, 33)), ((4096, 2049), (512, 257))],
+                         ids=["128x65-64x33", "4096x2049-512x257"])
+def test_once_per_step_closure_with_direct_calls_between(fine, coarse, progress):
     """Mode 2 (closure of each step's first stage state, held over the three stages) with direct
     nse2d_les_closure calls on other states between the steps: every direct call writes its
     output (the buffer is filled with NaN before each), repeated calls agree bit for bit, and each
     attached step equals the oracle step carrying the closure of that step's own first stage state
     as a fixed source (the stage-hook bars)."""
-    c = cfg(64, 33)
-    F = lx.Channel(cfg(128, 65), ic=False)
+    progress(f"[les] mode 2 closure on the {coarse} coarse channel (fine {fine})")
+    c = cfg(*coarse, Reb=2e4)
+    F = lx.Channel(cfg(*fine, Reb=2e4), ic=False)
     C = lx.Channel(c, ic=False)
     les = lx.Les(F, C, 1, 1, seed=3)
-    g = Grid(64, 33, 2.0)
+    g = Grid(coarse[0], coarse[1], 2.0)
+    Ny, Nx = coarse[1], coarse[0]
     mean = np.array([0.5, 1.0, 0.7, 0.0, 0.1])
     std = np.array([3.0, 0.6, 0.5, 0.2, 2.0])
     assert nh.lib().nse2d_les_set_scales(les.h, mean.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
@@ -251,8 +263,8 @@ def test_once_per_step_closure_with_direct_calls_between():
     les.attach(C, mode=2)
     other_w, other_psi = fine_state(g, 2.0, 9)
     d_ow, d_opsi = nh.DeviceBuffer.from_array(other_w), nh.DeviceBuffer.from_array(other_psi)
-    d_out = nh.DeviceArray(shape=(33, 64), dtype=np.float32)
-    nan = np.full((33, 64), np.nan, np.float32)
+    d_out = nh.DeviceArray(shape=(Ny, Nx), dtype=np.float32)
+    nan = np.full((Ny, Nx), np.nan, np.float32)
 
     def direct(wp, pp):
         d_out.upload(nan)
@@ -274,9 +286,12 @@ def test_once_per_step_closure_with_direct_calls_between():
             psi_o, w_o = o.step(psi_n.copy(), w_n.copy(), t, dt)
             e = step_errors(g, C.omega(), nh.download_ptr(C.field(nh.FIELD_PSI), g.Nx * g.Ny, (g.Ny, g.Nx)),
                             w_o, psi_o, 2.0)
-            print(f"[les] mode-2 step {k} with {k + 1} direct calls before it: psi {e[0]:.2e}, interior "
-                  f"{e[1]:.2e}, wall vs own closure {e[2]:.2e}", flush=True)
-            assert e[0] <= 2e-13 and e[1] <= 2e-13 and e[2] <= 1e-13
+            progress(f"[les] mode-2 step {k} with {k + 1} direct calls before it: psi {e[0]:.2e}, interior "
+                     f"{e[1]:.2e}, wall vs own closure {e[2]:.2e}")
+            # whole-step bars of test_gpu_step: at 257 rows the Thom coefficient (~3e6) carries the
+            # solvers' near-wall psi rounding into the wall vorticity and the next stage's interior
+            bar_int = 2e-13 if coarse[1] <= 65 else 5e-12
+            assert e[0] <= 2e-13 and e[1] <= bar_int and e[2] <= 1e-13
             t += dt
     finally:
         les.attach(C, mode=0)
