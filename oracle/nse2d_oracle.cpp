@@ -1237,6 +1237,19 @@ void write_snapshot(const ORun& R, double t, const std::vector<std::pair<std::st
     for (size_t k = 0; k < fields.size(); ++k) j << (k ? ", " : "") << "\"" << fields[k].first << "\"";
     j << (fields.empty() ? "" : ", ") << "\"y\"], \"dtype\": \"<f8\"}\n";
 }
+// The time "t" of a snapshot header (path with or without the .json/.bin suffix).
+bool read_snapshot_time(std::string base, double* t) {
+    for (const char* ext : {".json", ".bin"})
+        if (base.size() > std::strlen(ext) && base.substr(base.size() - std::strlen(ext)) == ext)
+            base = base.substr(0, base.size() - std::strlen(ext));
+    std::ifstream j(base + ".json");
+    if (!j) return false;
+    std::string s((std::istreambuf_iterator<char>(j)), std::istreambuf_iterator<char>());
+    const size_t p = s.find("\"t\"");
+    if (p == std::string::npos) return false;
+    *t = std::atof(s.c_str() + s.find(':', p) + 1);
+    return true;
+}
 bool read_snapshot_field(const std::string& base, const std::string& name, int Nx, int Ny, double* out) {
     std::ifstream j(base + ".json");
     if (!j) return false;
@@ -1481,7 +1494,7 @@ double sched_tol(double t, double dt) {
     const double tol_dt = 1e-6 * std::max(1e-300, std::fabs(dt));
     return std::max(tol_abs, tol_dt);
 }
-[[maybe_unused]] double next_after(double t0, double dt) {
+double next_after(double t0, double dt) {  // io.cu:173-178
     if (dt <= 0.0) return 1e300;
     const double k = std::floor((t0 + time_tol(t0)) / dt);
     const double t1 = (k + 1.0) * dt;
@@ -1679,10 +1692,20 @@ int run_main_impl(int argc, char** argv) {
         std::ofstream pl(R.outdir + "/progress.log", std::ios::out);
         pl << "oracle run Nx=" << G.Nx << " Ny=" << G.Ny << "\n";
     }
-    write_diag_header(diag_path, P.closure_diag);
+    // main.cu:160-165: the header only into an empty file (append-safe on resume)
+    {
+        std::ifstream chk(diag_path, std::ios::binary | std::ios::ate);
+        if (!chk.good() || chk.tellg() <= 0) write_diag_header(diag_path, P.closure_diag);
+    }
     { std::ofstream f(prof_path, std::ios::out); f << "j,y,yplus,U,V,uu,vv,uv,Uplus,tau_total\n"; }
     const size_t Ny = (size_t)G.Ny;
-    const double t_start = 0.0;
+    // main.cu:129-147: a resumed run starts at the snapshot's time (its header's "t" in place of the
+    // HDF5 attribute); step and dt are not restored (the step counter starts at 0 and dt at dt_init)
+    double t_start = 0.0;
+    if (P.resume && !read_snapshot_time(P.load_path, &t_start)) {
+        std::fprintf(stderr, "oracle: --resume needs a readable --load_path snapshot\n");
+        return 1;
+    }
     auto snap_fields = [&](std::vector<double>& u, std::vector<double>& v) {
         std::vector<std::pair<std::string, const double*>> fs;
         fs.push_back({"omega", S.w.data()});
@@ -1696,11 +1719,11 @@ int run_main_impl(int argc, char** argv) {
     };
     std::vector<double> ubuf(S.N), vbuf(S.N);
     const bool use_snap = (R.snap_save_dt > 0), use_diag = (R.diag_save_dt > 0);
-    double next_snap_t = use_snap ? 0.0 : 1e300;
-    double next_diag_t = use_diag ? 0.0 : 1e300;
+    double next_snap_t = use_snap ? (P.resume ? next_after(t_start, R.snap_save_dt) : 0.0) : 1e300;  // main.cu:313
+    double next_diag_t = use_diag ? (P.resume ? next_after(t_start, R.diag_save_dt) : 0.0) : 1e300;  // main.cu:314
     double tauw_abs_dt_sum = 0.0, T_diag = 0.0, last_diag_t = t_start;
 
-    if (use_snap) {  // main.cu:321-324
+    if (use_snap && !P.resume) {  // main.cu:321-324 (t = 0 outputs only for new runs)
         write_snapshot(R, 0.0, snap_fields(ubuf, vbuf));
         next_snap_t = R.snap_save_dt;
     }
@@ -1786,7 +1809,7 @@ int run_main_impl(int argc, char** argv) {
         }
         f << "\n";
     };
-    if (use_diag) {  // t = 0 row, main.cu:325-421
+    if (use_diag && !P.resume) {  // t = 0 row, main.cu:325-421
         S.build_rhs(S.w, t_start);
         const double mixed = orc_omega_rhs_balance_mapped(S.w.data(), S.rhs.data(), G.Nx, G.Ny, G.w_node.data());
         diag_row(0, 0.0, P.dt_init, true, mixed);

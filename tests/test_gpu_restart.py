@@ -11,6 +11,7 @@ import pytest
 
 import harness
 import nse2d_hip as nh
+import oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -44,3 +45,34 @@ def test_resume_equals_continuous_run(tmp_path, free_slip):
     # the resumed run's rows continue the whole run's rows at the same times and values
     assert int(db["step"][-1]) == int(da["step"][-1]) - 20
     np.testing.assert_array_equal(db["K"][-1], da["K"][-1])
+
+
+@pytest.mark.parametrize("free_slip", [False, True])
+def test_resume_matches_oracle_resume(tmp_path, free_slip):
+    """The HIP driver and the oracle driver resumed from the same snapshot (main.cu:129-147,
+    init.cu:215-279): the same rows at the CSV's 9 digits and the same final state within 1e-10.
+    Neither restores the step counter or dt (the reference restores neither)."""
+    print(f"[restart] HIP resume vs oracle resume (free_slip={free_slip})", flush=True)
+    extra = ["--walls", "freeslip"] if free_slip else []
+    a = str(tmp_path / "whole")
+    os.makedirs(a)
+    assert nh.run_main(BASE + extra + ["--tend", "0.02", "--snap_save", "0.02", "--outdir", a]) == 0
+    mid = os.path.join(a, "snaps", "snap_t0.020000.json")
+    out = {}
+    for name, run in (("hip", nh.run_main), ("oracle", oracle.run_main)):
+        d = str(tmp_path / name)
+        os.makedirs(d)
+        assert run(BASE + extra + ["--tend", "0.04", "--snap_save", "0.02", "--resume", "--init", "load",
+                                   "--load_path", mid, "--outdir", d]) == 0
+        out[name] = d
+    dh, do = harness.read_diag(out["hip"]), harness.read_diag(out["oracle"])
+    assert len(dh) == len(do) >= 2
+    np.testing.assert_array_equal(dh["step"], do["step"])
+    assert dh["step"][0] == 10 and dh["t"][0] > 0.02  # no t = 0 row, step counter from 0
+    bad = harness.compare_rows(do, dh, rel=5e-8)
+    assert not bad, bad[:5]
+    sh, so = harness.last_snap(out["hip"], 0.04), harness.last_snap(out["oracle"], 0.04)
+    e_w = np.linalg.norm(sh.omega - so.omega) / np.linalg.norm(so.omega)
+    e_p = np.linalg.norm(sh.psi - so.psi) / np.linalg.norm(so.psi)
+    print(f"[restart] resumed HIP vs resumed oracle: omega {e_w:.2e} psi {e_p:.2e}", flush=True)
+    assert e_w <= 1e-10 and e_p <= 1e-10
