@@ -298,12 +298,75 @@ def run_cnn(a):
     print(json.dumps(out), flush=True)
 
 
+def run_diag(a):
+    """One line of JSON: one diagnostics row (the reductions of main.cu:565-692) at 4096x2049 on the
+    channel's state after the warm-up, fused (nse2d_diag_row: one pass over psi, omega and rhs)
+    against the separate launchers, with the fused row's HBM roofline (algorithmic bytes: psi,
+    omega and rhs read once, 24 B per point)."""
+    L = nh.lib()
+    c = CFG
+    Nx, Ny = c["Nx"], c["Ny"]
+    ch, dm = setup_channel(c)
+    L.nse2d_channel_step_auto(ch, a.warmup)
+    L.nse2d_channel_step(ch, 0.0, 1e-5, nh.STEP_STORE_RHS)
+    psi, w, rhs = (L.nse2d_channel_field(ch, k) for k in (nh.FIELD_PSI, nh.FIELD_OMEGA, nh.FIELD_RHS))
+    inv_dx, inv_deta = Nx / c["Lx"], (Ny - 1) / 2.0
+    out = nh.DeviceBuffer(32)
+    sep = nh.DeviceBuffer(32)
+    s = L.nse2d_stream_create()
+    an, ae, yn, wn = dm["a_node"].ptr, dm["a_edge"].ptr, dm["y"].ptr, dm["w_node"].ptr
+
+    def fused():
+        if L.nse2d_diag_row(psi, w, rhs, Nx, Ny, inv_dx, inv_deta, c["h"], c["F0"], c["nforce"], an, ae, yn, wn,
+                            out.ptr, None, s):
+            raise RuntimeError(nh.last_error())
+
+    def separate():
+        b = sep.ptr
+        L.launch_max_uv_from_psi_mapped(psi, Nx, Ny, inv_dx, inv_deta, an, b, b + 8, b + 16, s)
+        L.launch_kinetic_energy_mapped(psi, Nx, Ny, inv_dx, inv_deta, an, wn, b + 24, s)
+        L.launch_enstrophy_diss_mapped_full(w, Nx, Ny, inv_dx, inv_deta, an, wn, b + 32, b + 40, s)
+        L.launch_omega_rhs_balance_mapped(w, rhs, Nx, Ny, wn, b + 48, s)
+        L.launch_power_input_mapped(psi, Nx, Ny, inv_deta, c["h"], c["F0"], c["nforce"], an, yn, wn, b + 56, s)
+        L.launch_energy_dissipation_mapped(psi, w, Nx, Ny, inv_dx, inv_deta, an, ae, wn, b + 64, s)
+        L.launch_enstrophy_input_mapped_full(w, Nx, Ny, c["h"], c["F0"], c["nforce"], yn, wn, b + 72, s)
+        L.launch_mean_wall_omega_omegay_mapped(w, Nx, Ny, inv_deta, an, b + 80, b + 88, s)
+        L.launch_mean_wall_omega(w, Nx, Ny, b + 96, b + 104, s)
+
+    res = {}
+    for name, fn in (("fused", fused), ("separate", separate)):
+        for _ in range(5):
+            fn()
+        L.nse2d_stream_sync(s)
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            fn()
+        L.nse2d_stream_sync(s)
+        res[name] = (time.perf_counter() - t0) / a.steps
+    same = bool(np.array_equal(out.download()[:14], sep.download()[:14]))
+    L.nse2d_stream_destroy(s)
+    L.nse2d_channel_destroy(ch)
+    alg = 24.0 * Nx * Ny
+    achieved = alg / res["fused"] / 1e9
+    print(json.dumps({
+        "metric": "diagnostics row time at 4096x2049 FP64 (fused one-pass reductions)", "value": res["fused"] * 1e6,
+        "unit": "us", "higher_is_better": False, "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (the channel's state after the warm-up)",
+        "config": {"workload": "one diagnostics row of the 4096x2049 forced channel: CFL maxima, K, Omega, eta, "
+                               "<omega rhs>, Pin, epsilon, P_Omega, wall means"},
+        "separate_launchers_us": res["separate"] * 1e6, "speedup": res["separate"] / res["fused"],
+        "fused_equals_separate_bitwise": same,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "alg_bytes_per_row": alg,
+                     "note": "host-timed per row (two launches), so launch gaps are included"}}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["channel", "ensemble", "les", "cnn"], default="channel",
+    ap.add_argument("--workload", choices=["channel", "ensemble", "les", "cnn", "diag"], default="channel",
                     help="channel: the 4096x2049 forced channel (default); ensemble: 64 independent channels; "
                          "les: learned-closure LES at 512x257 trained on the 4096x2049 DNS; cnn: the closure "
-                         "network alone")
+                         "network alone; diag: one fused diagnostics row")
     ap.add_argument("--les-spinup", type=float, default=4.0)
     ap.add_argument("--les-window", type=float, default=4.0)
     ap.add_argument("--les-pairs", type=int, default=2000)
@@ -342,6 +405,9 @@ def main():
         return
     if a.workload == "cnn":
         run_cnn(a)
+        return
+    if a.workload == "diag":
+        run_diag(a)
         return
     c = CFG
     Nx, Ny = c["Nx"], c["Ny"]

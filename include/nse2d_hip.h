@@ -215,6 +215,24 @@ void launch_omega_rhs_balance_mapped(const double* d_omega, const double* d_rhs,
 void launch_max_uv_from_psi_mapped(const double* d_psi, int Nx, int Ny, double inv_dx,
                                    double inv_deta, const double* d_a_node, double* d_umax,
                                    double* d_vmax, double* d_vmaxm, hipStream_t stream);
+/* One diagnostics row (main.cu:565-692 without --closure_diag; no single reference counterpart):
+ * the reductions of launch_max_uv_from_psi_mapped, launch_kinetic_energy_mapped,
+ * launch_enstrophy_diss_mapped_full, launch_omega_rhs_balance_mapped, launch_power_input_mapped,
+ * launch_energy_dissipation_mapped, launch_enstrophy_input_mapped_full,
+ * launch_mean_wall_omega_omegay_mapped and launch_mean_wall_omega, and optionally
+ * launch_row_moments_from_psi_mapped (d_moments: [5][Ny] row sums of u, v, u^2, v^2, uv, or
+ * NULL), in one pass over psi, omega and rhs (two launches). d_out receives NSE2D_DIAG_NOUT
+ * doubles, each equal to the corresponding launcher's output bit for bit. 0, or -1 for Nx < 2,
+ * Ny < 4 or a null field or output pointer. */
+enum { NSE2D_DIAG_UMAX = 0, NSE2D_DIAG_VMAX = 1, NSE2D_DIAG_VMAXM = 2, NSE2D_DIAG_K = 3,
+       NSE2D_DIAG_OMEGA_W2 = 4, NSE2D_DIAG_OMEGA_GRAD2 = 5, NSE2D_DIAG_OMEGA_RHS = 6,
+       NSE2D_DIAG_PIN = 7, NSE2D_DIAG_EPS = 8, NSE2D_DIAG_POMEGA = 9,
+       NSE2D_DIAG_WALL_OMEGA_OMEGAY_BOT = 10, NSE2D_DIAG_WALL_OMEGA_OMEGAY_TOP = 11,
+       NSE2D_DIAG_WALL_OMEGA_BOT = 12, NSE2D_DIAG_WALL_OMEGA_TOP = 13, NSE2D_DIAG_NOUT = 14 };
+int nse2d_diag_row(const double* d_psi, const double* d_omega, const double* d_rhs, int Nx, int Ny,
+                   double inv_dx, double inv_deta, double h, double F0, int nforce,
+                   const double* d_a_node, const double* d_a_edge, const double* d_y_node,
+                   const double* d_w_node, double* d_out, double* d_moments, hipStream_t stream);
 /* No counterpart in the reference. launch_max_uv_from_psi_mapped for callers that cannot
  * allocate device memory: code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc. Same kernels and results; the row partials live in a fixed device array of

@@ -214,6 +214,24 @@ struct OmegaRhs {
     }
 };
 
+// The five row-moment sums of one point (diag.cu:468-499: u with clamped y neighbours, centred v),
+// with the multiply-adds written out and no other contraction, so that every kernel using this
+// rounds it alike.
+__device__ __forceinline__ void moments_point(const double* __restrict__ psi, int Nx, int ix, int iy, int ym, int yp,
+                                              double inv_dx, double inv_deta, double a, double (&acc)[5]) {
+    // no contraction of u = d * a into the sum acc[0] + u: whether the compiler would fuse it
+    // depends on the other uses of u in the calling kernel
+#pragma clang fp contract(off)
+    const int xm = (ix == 0 ? Nx - 1 : ix - 1), xp = (ix == Nx - 1 ? 0 : ix + 1);
+    const double u = (psi[idx2(yp, ix, Nx)] - psi[idx2(ym, ix, Nx)]) * 0.5 * inv_deta * a;
+    const double v = -(psi[idx2(iy, xp, Nx)] - psi[idx2(iy, xm, Nx)]) * 0.5 * inv_dx;
+    acc[0] += u;
+    acc[1] += v;
+    acc[2] = __builtin_fma(u, u, acc[2]);
+    acc[3] = __builtin_fma(v, v, acc[3]);
+    acc[4] = __builtin_fma(u, v, acc[4]);
+}
+
 // ---- diag.cu:468-499: one workgroup per row, five sums, clamped wall rows
 __global__ void __launch_bounds__(RB) row_moments_kernel(const double* __restrict__ psi, int Nx, int Ny,
                                                          double inv_dx, double inv_deta,
@@ -223,14 +241,20 @@ __global__ void __launch_bounds__(RB) row_moments_kernel(const double* __restric
     const int iy = blockIdx.x;
     double acc[5] = {0, 0, 0, 0, 0};
     const int ym = (iy == 0 ? 0 : iy - 1), yp = (iy == Ny - 1 ? Ny - 1 : iy + 1);
-    for (int ix = threadIdx.x; ix < Nx; ix += RB) {
-        const int xm = (ix == 0 ? Nx - 1 : ix - 1), xp = (ix == Nx - 1 ? 0 : ix + 1);
-        const double u = (psi[idx2(yp, ix, Nx)] - psi[idx2(ym, ix, Nx)]) * 0.5 * inv_deta * a_node[iy];
-        const double v = -(psi[idx2(iy, xp, Nx)] - psi[idx2(iy, xm, Nx)]) * 0.5 * inv_dx;
-        acc[0] += u; acc[1] += v; acc[2] += u * u; acc[3] += v * v; acc[4] += u * v;
-    }
+    const double a = a_node[iy];
+    for (int ix = threadIdx.x; ix < Nx; ix += RB) moments_point(psi, Nx, ix, iy, ym, yp, inv_dx, inv_deta, a, acc);
     block_sum<5>(acc, sm);
     if (threadIdx.x == 0) { su[iy] = acc[0]; sv[iy] = acc[1]; su2[iy] = acc[2]; sv2[iy] = acc[3]; suv[iy] = acc[4]; }
+}
+
+// omega * a * omega_eta on a wall row (diag.cu:516-583): second-order one-sided eta derivative
+// into the channel (rows 0, 1, 2 at the bottom, Ny-1, Ny-2, Ny-3 at the top). The multiply-adds
+// are written out, so that every kernel using this rounds it alike.
+__device__ __forceinline__ double wall_omega_omegay(double w0, double w1, double w2, double a, double half_inv_deta,
+                                                    bool bottom) {
+#pragma clang fp contract(off)
+    const double t = bottom ? __builtin_fma(-3.0, w0, 4.0 * w1) - w2 : __builtin_fma(3.0, w0, -(4.0 * w1)) + w2;
+    return w0 * (a * (t * half_inv_deta));
 }
 
 // ---- diag.cu:500-512 and 516-583: x-means over the two wall rows (one workgroup)
@@ -243,10 +267,9 @@ __global__ void __launch_bounds__(RB) wall_means_kernel(const double* __restrict
     for (int ix = threadIdx.x; ix < Nx; ix += RB) {
         const double w0 = w[idx2(0, ix, Nx)], wN = w[idx2(Ny - 1, ix, Nx)];
         if (WITH_GRAD) {
-            const double weta_bot = (-3.0 * w0 + 4.0 * w[idx2(1, ix, Nx)] - 1.0 * w[idx2(2, ix, Nx)]) * (0.5 * inv_deta);
-            const double weta_top = (3.0 * wN - 4.0 * w[idx2(Ny - 2, ix, Nx)] + 1.0 * w[idx2(Ny - 3, ix, Nx)]) * (0.5 * inv_deta);
-            acc[0] += w0 * (a_node[0] * weta_bot);
-            acc[1] += wN * (a_node[Ny - 1] * weta_top);
+            acc[0] += wall_omega_omegay(w0, w[idx2(1, ix, Nx)], w[idx2(2, ix, Nx)], a_node[0], 0.5 * inv_deta, true);
+            acc[1] += wall_omega_omegay(wN, w[idx2(Ny - 2, ix, Nx)], w[idx2(Ny - 3, ix, Nx)], a_node[Ny - 1],
+                                        0.5 * inv_deta, false);
         } else {
             acc[0] += w0;
             acc[1] += wN;
@@ -296,6 +319,135 @@ __global__ void __launch_bounds__(RB) final_max_group_kernel(const MaxUVJob* __r
 
 __global__ void zero3_kernel(double* a, double* b, double* c) {
     if (threadIdx.x == 0) { *a = 0.0; *b = 0.0; *c = 0.0; }
+}
+
+// ---- One diagnostics row in one pass (main.cu:565-692 without --closure_diag): workgroup iy reads
+// rows iy-1..iy+1 of psi and omega (the neighbour rows hit in L2) and row iy of rhs, and forms the
+// row partials of every reduction of the row with the per-point bodies of the launchers above,
+// each accumulator in the same order as its own launcher's row kernel; the wall rows' workgroups
+// also form the wall means. The final kernel then adds (or, for the CFL maxima, takes the maximum
+// of) the row partials in row order, as the launchers' final kernels do: every output equals the
+// separate launchers' bit for bit. Outputs: NSE2D_DIAG_* of include/nse2d_hip.h.
+struct DiagRowArgs {
+    const double *psi, *omega, *rhs;
+    int Nx, Ny;
+    double inv_dx, inv_deta, h, F0;
+    int nforce;
+    const double *a_node, *a_edge, *y_node, *w_node;
+    double *out, *moments;  // moments: [5][Ny] row sums of u, v, u^2, v^2, uv, or nullptr
+};
+
+constexpr int kDiagSums = 7;  // K, W2, GW2, omega*rhs, Pin, eps, P_Omega
+
+__global__ void __launch_bounds__(RB) diag_row_kernel(DiagRowArgs A, double* __restrict__ partial) {
+    __shared__ double sm_s[kDiagSums * 16], sm_m[3 * 16], sm_w[5 * 16];
+    const int iy = blockIdx.x, Nx = A.Nx, Ny = A.Ny;
+    const KineticEnergy ke{A.psi, Nx, Ny, A.inv_dx, A.inv_deta, A.a_node, A.w_node};
+    const EnstDissFull ed{A.omega, Nx, Ny, A.inv_dx, A.inv_deta, A.a_node, A.w_node};
+    const OmegaRhs orh{A.omega, A.rhs, Nx, Ny, A.w_node};
+    const PowerInput pin{A.psi, Nx, Ny, A.inv_deta, A.h, A.F0, A.nforce, A.a_node, A.y_node, A.w_node};
+    const EnergyDiss eps{A.psi, A.omega, Nx, Ny, A.inv_dx, A.inv_deta, A.a_node, A.a_edge, A.w_node};
+    const EnstInput pom{A.omega, Nx, Ny, A.h, A.F0, A.nforce, A.y_node, A.w_node, true};
+    const bool interior = iy > 0 && iy < Ny - 1;
+    double k1[1] = {0.0}, e2[2] = {0.0, 0.0}, o1[1] = {0.0}, p1[1] = {0.0}, s1[1] = {0.0}, q1[1] = {0.0};
+    double m[3] = {0.0, 0.0, 0.0};
+    double mo[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    const int ym = (iy == 0 ? 0 : iy - 1), yp = (iy == Ny - 1 ? Ny - 1 : iy + 1);
+    const double a = A.a_node[iy];
+    for (int ix = threadIdx.x; ix < Nx; ix += RB) {
+        if (interior) {
+            ke(ix, iy, k1);
+            // row_max_uv_body (diag.cu:702-866)
+            const int ip = (ix + 1 < Nx) ? (ix + 1) : 0, im = (ix > 0) ? (ix - 1) : (Nx - 1);
+            const double dpsideta = (A.psi[idx2(iy + 1, ix, Nx)] - A.psi[idx2(iy - 1, ix, Nx)]) * (0.5 * A.inv_deta);
+            const double dpsidx = (A.psi[idx2(iy, ip, Nx)] - A.psi[idx2(iy, im, Nx)]) * (0.5 * A.inv_dx);
+            const double au = fabs(a * dpsideta), av = fabs(-dpsidx);
+            m[0] = fmax(m[0], au);
+            m[1] = fmax(m[1], av);
+            m[2] = fmax(m[2], av * a);
+        }
+        ed(ix, iy, e2);
+        orh(ix, iy, o1);
+        pin(ix, iy, p1);
+        eps(ix, iy, s1);
+        pom(ix, iy, q1);
+        if (A.moments) moments_point(A.psi, Nx, ix, iy, ym, yp, A.inv_dx, A.inv_deta, a, mo);  // diag.cu:468-499
+    }
+    double sums[kDiagSums] = {k1[0], e2[0], e2[1], o1[0], p1[0], s1[0], q1[0]};
+    block_sum<kDiagSums>(sums, sm_s);
+    block_max<3>(m, sm_m);
+    if (A.moments) block_sum<5>(mo, sm_w);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < kDiagSums; ++k) partial[(size_t)k * Ny + iy] = sums[k];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) partial[(size_t)(kDiagSums + k) * Ny + iy] = m[k];
+        if (A.moments)
+#pragma unroll
+            for (int k = 0; k < 5; ++k) A.moments[(size_t)k * Ny + iy] = mo[k];
+    }
+    if (iy != 0 && iy != Ny - 1) return;  // uniform over the workgroup
+    // wall means (diag.cu:500-512, 516-583), one wall per workgroup
+    __syncthreads();  // sm_w is reused
+    const bool bot = (iy == 0);
+    double wa[2] = {0.0, 0.0};
+    for (int ix = threadIdx.x; ix < Nx; ix += RB) {
+        const double w0 = A.omega[idx2(bot ? 0 : Ny - 1, ix, Nx)];
+        const double w1 = A.omega[idx2(bot ? 1 : Ny - 2, ix, Nx)], w2 = A.omega[idx2(bot ? 2 : Ny - 3, ix, Nx)];
+        wa[0] += wall_omega_omegay(w0, w1, w2, A.a_node[bot ? 0 : Ny - 1], 0.5 * A.inv_deta, bot);
+        wa[1] += w0;
+    }
+    block_sum<2>(wa, sm_w);
+    if (threadIdx.x == 0) {
+        A.out[bot ? NSE2D_DIAG_WALL_OMEGA_OMEGAY_BOT : NSE2D_DIAG_WALL_OMEGA_OMEGAY_TOP] = wa[0] / (double)Nx;
+        A.out[bot ? NSE2D_DIAG_WALL_OMEGA_BOT : NSE2D_DIAG_WALL_OMEGA_TOP] = wa[1] / (double)Nx;
+    }
+}
+
+// The row partials of diag_row_kernel in row order: sums, then the CFL maxima.
+__global__ void __launch_bounds__(RB) diag_final_kernel(const double* __restrict__ partial, int n,
+                                                        double* __restrict__ out) {
+    __shared__ double sm_s[kDiagSums * 16], sm_m[3 * 16];
+    double s[kDiagSums], m[3];
+#pragma unroll
+    for (int a = 0; a < kDiagSums + 3; ++a) {
+        if (a < kDiagSums) s[a] = 0.0;
+        else m[a - kDiagSums] = 0.0;
+    }
+    // the partials of this thread (rows tid, tid + RB, ...) added in row order, as final_kernel
+    // does, with eight loads issued before their additions
+    constexpr int B = 8;
+    for (int i0 = threadIdx.x; i0 < n; i0 += B * RB) {
+#pragma unroll
+        for (int a = 0; a < kDiagSums + 3; ++a) {
+            double v[B];
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                const int i = i0 + b * RB;
+                v[b] = (i < n) ? partial[(size_t)a * n + i] : 0.0;
+            }
+#pragma unroll
+            for (int b = 0; b < B; ++b) {
+                if (i0 + b * RB >= n) break;
+                if (a < kDiagSums) s[a] += v[b];
+                else m[a - kDiagSums] = fmax(m[a - kDiagSums], v[b]);
+            }
+        }
+    }
+    block_sum<kDiagSums>(s, sm_s);
+    block_max<3>(m, sm_m);
+    if (threadIdx.x == 0) {
+        out[NSE2D_DIAG_K] = s[0];
+        out[NSE2D_DIAG_OMEGA_W2] = s[1];
+        out[NSE2D_DIAG_OMEGA_GRAD2] = s[2];
+        out[NSE2D_DIAG_OMEGA_RHS] = s[3];
+        out[NSE2D_DIAG_PIN] = s[4];
+        out[NSE2D_DIAG_EPS] = s[5];
+        out[NSE2D_DIAG_POMEGA] = s[6];
+        out[NSE2D_DIAG_UMAX] = m[0];
+        out[NSE2D_DIAG_VMAX] = m[1];
+        out[NSE2D_DIAG_VMAXM] = m[2];
+    }
 }
 
 }  // namespace
@@ -469,6 +621,24 @@ void launch_max_uv_from_psi_mapped(const double* d_psi, int Nx, int Ny, double i
                                    const double* d_a_node, double* d_umax, double* d_vmax, double* d_vmaxm,
                                    hipStream_t stream) {
     launch_max_uv(d_psi, Nx, Ny, inv_dx, inv_deta, d_a_node, d_umax, d_vmax, d_vmaxm, stream);
+}
+
+int nse2d_diag_row(const double* d_psi, const double* d_omega, const double* d_rhs, int Nx, int Ny, double inv_dx,
+                   double inv_deta, double h, double F0, int nforce, const double* d_a_node, const double* d_a_edge,
+                   const double* d_y_node, const double* d_w_node, double* d_out, double* d_moments,
+                   hipStream_t stream) {
+    if (Nx < 2 || Ny < 4 || !d_psi || !d_omega || !d_rhs || !d_out) {
+        set_last_error("nse2d_diag_row: needs Nx >= 2, Ny >= 4 and the psi, omega, rhs and output pointers");
+        return -1;
+    }
+    double* partial = scratch_doubles((size_t)(kDiagSums + 3) * Ny);
+    const DiagRowArgs A{d_psi, d_omega, d_rhs, Nx, Ny, inv_dx, inv_deta, h, F0, nforce,
+                        d_a_node, d_a_edge, d_y_node, d_w_node, d_out, d_moments};
+    diag_row_kernel<<<Ny, RB, 0, stream>>>(A, partial);
+    NSE2D_LAUNCH_CHECK();
+    diag_final_kernel<<<1, RB, 0, stream>>>(partial, Ny, d_out);
+    NSE2D_LAUNCH_CHECK();
+    return 0;
 }
 
 int launch_max_uv_from_psi_mapped_noalloc(const double* d_psi, int Nx, int Ny, double inv_dx, double inv_deta,

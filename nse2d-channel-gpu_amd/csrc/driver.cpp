@@ -525,8 +525,13 @@ T* dalloc(size_t n) {
 
 // Slots of the diagnostics scalar array (one D2H copy per row).
 enum Slot {
-    S_UMAX, S_VMAX, S_VMAXM, S_K, S_W2, S_GW2, S_WWYB, S_WWYT, S_PIN, S_EPS, S_POM, S_ORHS_MIXED, S_ORHS,
-    S_MWB, S_MWT, S_W2I, S_GW2I, S_T2D, S_T2J, S_T2W2, S_T2G2, S_T2P, S_T2ETA, S_T2PHI, S_NSLOTS
+    // the first NSE2D_DIAG_NOUT slots are nse2d_diag_row's outputs
+    S_UMAX = NSE2D_DIAG_UMAX, S_VMAX = NSE2D_DIAG_VMAX, S_VMAXM = NSE2D_DIAG_VMAXM, S_K = NSE2D_DIAG_K,
+    S_W2 = NSE2D_DIAG_OMEGA_W2, S_GW2 = NSE2D_DIAG_OMEGA_GRAD2, S_ORHS_MIXED = NSE2D_DIAG_OMEGA_RHS,
+    S_PIN = NSE2D_DIAG_PIN, S_EPS = NSE2D_DIAG_EPS, S_POM = NSE2D_DIAG_POMEGA,
+    S_WWYB = NSE2D_DIAG_WALL_OMEGA_OMEGAY_BOT, S_WWYT = NSE2D_DIAG_WALL_OMEGA_OMEGAY_TOP,
+    S_MWB = NSE2D_DIAG_WALL_OMEGA_BOT, S_MWT = NSE2D_DIAG_WALL_OMEGA_TOP,
+    S_ORHS = NSE2D_DIAG_NOUT, S_W2I, S_GW2I, S_T2D, S_T2J, S_T2W2, S_T2G2, S_T2P, S_T2ETA, S_T2PHI, S_NSLOTS
 };
 
 int run(Run& R) {
@@ -676,14 +681,14 @@ int run(Run& R) {
 
     // One diagnostics row (main.cu:325-421 for t = 0, main.cu:565-692 afterwards).
     auto diag_row = [&](long long stepno, double t, double dt_col, bool t0) {
-        launch_max_uv_from_psi_mapped(d_psi, Nx, Ny, inv_dx, inv_deta, d_a_node, d_sc + S_UMAX, d_sc + S_VMAX,
-                                      d_sc + S_VMAXM, stream);
-        launch_kinetic_energy_mapped(d_psi, Nx, Ny, inv_dx, inv_deta, d_a_node, d_w_node, d_sc + S_K, stream);
-        launch_enstrophy_diss_mapped_full(d_w, Nx, Ny, inv_dx, inv_deta, d_a_node, d_w_node, d_sc + S_W2,
-                                          d_sc + S_GW2, stream);
-        launch_mean_wall_omega_omegay_mapped(d_w, Nx, Ny, inv_deta, d_a_node, d_sc + S_WWYB, d_sc + S_WWYT, stream);
         if (t0) rebuild_rhs(t_start);  // main.cu:345-362: the t = 0 row builds the RHS
-        launch_omega_rhs_balance_mapped(d_w, d_rhs, Nx, Ny, d_w_node, d_sc + S_ORHS_MIXED, stream);
+        // CFL maxima, K, Omega, eta, <omega rhs> (with the stored RHS), Pin, epsilon, P_Omega and the
+        // wall means in one pass (the launchers of main.cu:565-692, fused; equal bit for bit)
+        if (nse2d_diag_row(d_psi, d_w, d_rhs, Nx, Ny, inv_dx, inv_deta, G.h, P.F0, P.nforce, d_a_node, d_a_edge,
+                           d_y_node, d_w_node, d_sc, nullptr, stream) != 0) {
+            std::fprintf(stderr, "ERROR: %s\n", nse2d_last_error());
+            std::abort();
+        }
         if (!t0 && P.closure_diag) {  // main.cu:591-611
             rebuild_rhs(t);
             launch_omega_rhs_balance_mapped(d_w, d_rhs, Nx, Ny, d_w_node, d_sc + S_ORHS, stream);
@@ -698,13 +703,6 @@ int run(Run& R) {
             launch_enstrophy_sbp_mapped(d_w, Nx, Ny, inv_dx, inv_deta, d_a_edge, d_wt_node, d_sc + S_T2ETA,
                                         d_sc + S_T2PHI, stream);
         }
-        launch_power_input_mapped(d_psi, Nx, Ny, inv_deta, G.h, P.F0, P.nforce, d_a_node, d_y_node, d_w_node,
-                                  d_sc + S_PIN, stream);
-        launch_energy_dissipation_mapped(d_psi, d_w, Nx, Ny, inv_dx, inv_deta, d_a_node, d_a_edge, d_w_node,
-                                         d_sc + S_EPS, stream);
-        launch_enstrophy_input_mapped_full(d_w, Nx, Ny, G.h, P.F0, P.nforce, d_y_node, d_w_node, d_sc + S_POM,
-                                           stream);
-        launch_mean_wall_omega(d_w, Nx, Ny, d_sc + S_MWB, d_sc + S_MWT, stream);
         NSE2D_HIP_CHECK(hipMemcpyAsync(sc, d_sc, S_NSLOTS * sizeof(double), hipMemcpyDeviceToHost, stream));
         NSE2D_HIP_CHECK(hipStreamSynchronize(stream));
         // wall-row part of <omega rhs>_A at the same state (main.cu:290-296, 601)

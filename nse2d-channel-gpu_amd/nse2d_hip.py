@@ -65,6 +65,7 @@ SIGNATURES = {
     "launch_enstrophy_diss_balance_mapped": (None, [_vp, _i, _i, _d, _d, _vp, _vp, _vp, _vp, _vp, _stream]),
     "launch_omega_rhs_balance_mapped": (None, [_vp, _vp, _i, _i, _vp, _vp, _stream]),
     "launch_max_uv_from_psi_mapped": (None, [_vp, _i, _i, _d, _d, _vp, _vp, _vp, _vp, _stream]),
+    "nse2d_diag_row": (_i, [_vp, _vp, _vp, _i, _i, _d, _d, _d, _d, _i, _vp, _vp, _vp, _vp, _vp, _vp, _stream]),
     "launch_max_uv_from_psi_mapped_noalloc": (_i, [_vp, _i, _i, _d, _d, _vp, _vp, _vp, _vp, _stream]),
     # whole step
     "nse2d_channel_create": (_vp, [_vp, _stream]),

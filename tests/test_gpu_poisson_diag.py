@@ -234,6 +234,59 @@ def test_max_uv_and_row_moments(dcase):
         assert np.all(np.abs(o.download() - r) <= 1e-13 * a + 1e-300)
 
 
+def _launcher_row(g, dpsi, dw, drhs, h=1.0, F0=0.05, nforce=12):
+    """The outputs of nse2d_diag_row from the separate launchers, in the NSE2D_DIAG_* order."""
+    Nx, Ny, L = g.Nx, g.Ny, nh.lib()
+    o = scalars(16)
+    b = o.ptr
+    L.launch_max_uv_from_psi_mapped(dpsi.ptr, Nx, Ny, g.inv_dx, g.inv_deta, g.dp("a_node"), b, b + 8, b + 16, None)
+    L.launch_kinetic_energy_mapped(dpsi.ptr, Nx, Ny, g.inv_dx, g.inv_deta, g.dp("a_node"), g.dp("w_node"), b + 24, None)
+    L.launch_enstrophy_diss_mapped_full(dw.ptr, Nx, Ny, g.inv_dx, g.inv_deta, g.dp("a_node"), g.dp("w_node"), b + 32,
+                                        b + 40, None)
+    L.launch_omega_rhs_balance_mapped(dw.ptr, drhs.ptr, Nx, Ny, g.dp("w_node"), b + 48, None)
+    L.launch_power_input_mapped(dpsi.ptr, Nx, Ny, g.inv_deta, h, F0, nforce, g.dp("a_node"), g.dp("y"),
+                                g.dp("w_node"), b + 56, None)
+    L.launch_energy_dissipation_mapped(dpsi.ptr, dw.ptr, Nx, Ny, g.inv_dx, g.inv_deta, g.dp("a_node"), g.dp("a_edge"),
+                                       g.dp("w_node"), b + 64, None)
+    L.launch_enstrophy_input_mapped_full(dw.ptr, Nx, Ny, h, F0, nforce, g.dp("y"), g.dp("w_node"), b + 72, None)
+    L.launch_mean_wall_omega_omegay_mapped(dw.ptr, Nx, Ny, g.inv_deta, g.dp("a_node"), b + 80, b + 88, None)
+    L.launch_mean_wall_omega(dw.ptr, Nx, Ny, b + 96, b + 104, None)
+    return o.download()[:14]
+
+
+@pytest.mark.parametrize("Nx,Ny,beta", DIAG_GRIDS + [(4096, 2049, 2.0), (300, 5, 2.0)])
+def test_fused_diag_row_equals_launchers(Nx, Ny, beta):
+    """nse2d_diag_row (one pass over psi, omega and rhs) against the separate launchers of the
+    same reductions, bit for bit, including the optional row moments; the launchers themselves
+    are checked against the oracle above."""
+    print(f"[diag row] {Nx}x{Ny} beta={beta}", flush=True)
+    g = Grid(Nx, Ny, beta)
+    psi = smooth_random_field(Nx, Ny, 31, y=g.m.y)
+    w = smooth_random_field(Nx, Ny, 32, scale=2.0, y=g.m.y)
+    rhs = smooth_random_field(Nx, Ny, 33, y=g.m.y)
+    dpsi, dw, drhs = dev(psi), dev(w), dev(rhs)
+    ref = _launcher_row(g, dpsi, dw, drhs)
+    out, mom = scalars(14), zeros_dev(5 * Ny)
+    for moments in (None, mom.ptr):
+        assert nh.lib().nse2d_diag_row(dpsi.ptr, dw.ptr, drhs.ptr, Nx, Ny, g.inv_dx, g.inv_deta, 1.0, 0.05, 12,
+                                       g.dp("a_node"), g.dp("a_edge"), g.dp("y"), g.dp("w_node"), out.ptr, moments,
+                                       None) == 0, nh.last_error()
+        got = out.download()
+        bad = [k for k in range(14) if not got[k] == ref[k]]
+        assert not bad, [(k, got[k], ref[k]) for k in bad]
+    outs = [zeros_dev(Ny) for _ in range(5)]
+    nh.lib().launch_row_moments_from_psi_mapped(dpsi.ptr, Nx, Ny, g.inv_dx, g.inv_deta, g.dp("a_node"),
+                                                *[o.ptr for o in outs], None)
+    got_m = mom.download((5, Ny))
+    for k in range(5):
+        ref_m = outs[k].download()
+        diff = np.nonzero(got_m[k] != ref_m)[0]
+        assert diff.size == 0, (k, diff[:5], got_m[k][diff[:3]], ref_m[diff[:3]])
+    assert nh.lib().nse2d_diag_row(dpsi.ptr, dw.ptr, drhs.ptr, Nx, 3, g.inv_dx, g.inv_deta, 1.0, 0.05, 12,
+                                   g.dp("a_node"), g.dp("a_edge"), g.dp("y"), g.dp("w_node"), out.ptr, None,
+                                   None) == -1
+
+
 @pytest.mark.parametrize("Nx,Ny", [(64, 17), (512, 33), (4096, 65)])
 def test_spectra1d_matches_oracle(Nx, Ny):
     """nse2d_spectra1d (diag.cuh:123-191 on the row FFT) against the oracle's restatement, over
