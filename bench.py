@@ -8,7 +8,8 @@ adaptive dt chosen on the device).
 Prints one JSON line: throughput in grid-point steps per second, the roofline of the dominant
 kernel (HIP events on the launch stream during the timed steps, algorithmic bytes per launch) and
 the CPU baseline (the FP64 OpenMP oracle timed on a bounded sample of the same workload on this
-host's cores).
+host's cores), and under "les" the learned-closure LES leg (512x257 against the 4096x2049 DNS,
+about 3 minutes, run in a child process; --no-les leaves it out).
 """
 import argparse
 import ctypes
@@ -222,38 +223,80 @@ def run_ensemble(a):
     print(json.dumps(out), flush=True)
 
 
+LES_CONFIG = ("learned-closure LES 512x257 (tanh beta=2, Re_b=2e4, F0=0.05 n=12, drag 1e-3) trained on "
+              "4096x2049 DNS filtered 8x in x (sharp spectral) and 8x in mapped y (box)")
+
+
+def les_record(r, a):
+    """The LES leg's figures: speedup per unit simulated time of the coarse run with the closure
+    evaluated once per step, and for each coarse run the Re_tau, U(y), budget and trajectory
+    errors against the filtered DNS, over the training window and over the whole comparison."""
+    runs = {}
+    for name in ("closure_per_step", "closure_mean_free", "no_closure"):
+        x = r["runs"][name]
+        w = x.get("window", {})
+        runs[name] = {
+            "finite": x["finite"],
+            "re_tau_err_window": w.get("re_tau_rel_err"), "U_err_window": w.get("U_rel_err"),
+            "budget_err_window": w.get("budget"), "traj_err_window_end": x.get("traj_err_window_end"),
+            "re_tau_err": x.get("re_tau_rel_err"), "U_err": x.get("U_rel_err"), "budget_err": x.get("budget"),
+            "re_tau_err_thirds": x.get("re_tau_rel_err_thirds"), "traj_err_end": x.get("traj_err_end"),
+            "wall_s_per_unit_time": x["wall_per_time"],
+            "within_2pct": bool(x["finite"] and (x.get("re_tau_rel_err") or 1.0) <= 0.02
+                                and (x.get("U_rel_err") or 1.0) <= 0.02)}
+    t0 = a.les_spinup
+    return {
+        "metric": "LES wall-clock speedup over DNS per unit simulated time (512x257 closure LES vs 4096x2049 DNS)",
+        "speedup": r["speedup_vs_dns_per_step_closure"], "unit": "x",
+        "speedup_closure_every_stage": r["speedup_vs_dns"],
+        "dns_wall_s_per_unit_time": r["dns_wall_per_time"],
+        "config": {"workload": LES_CONFIG, "pairs": a.les_pairs, "batch": 32, "train_steps": a.les_train_steps,
+                   "training_window": [t0, t0 + a.les_window], "comparison_window": [t0, t0 + a.les_compare],
+                   "closure": "evaluated once per step, held over the three stages"},
+        "re_tau_dns_filtered": r["re_tau_dns_filtered"], "re_tau_dns_fine": r["re_tau_dns_fine"],
+        "holdout_mse_cnn": r["holdout_mse_cnn"], "holdout_mse_zero": r["holdout_mse_zero"],
+        "runs": runs,
+    }
+
+
 def run_les(a):
     """One line of JSON: the learned-closure LES at 512x257 against the 4096x2049 DNS it is trained
     on (DNS -> 2000 filtered pairs -> Adam, batch 32 -> coarse runs with and without the closure
-    from the filtered DNS state): wall-clock speedup per unit of simulated time, Re_tau and U(y)
-    errors against the filtered DNS over the same window."""
+    from the filtered DNS state): wall-clock speedup per unit of simulated time, Re_tau, U(y),
+    budget and trajectory errors against the filtered DNS."""
     import les_experiment as lx
     fine = dict(CFG)
     coarse = dict(CFG, Nx=512, Ny=257)
     r = lx.run(fine, coarse, spinup_time=a.les_spinup, window_time=a.les_window, n_pairs=a.les_pairs,
                holdout=max(1, a.les_pairs // 20), train_steps=a.les_train_steps, batch=32, lr=1e-3,
-               n_samples=min(100, a.les_pairs), log=lambda m: print("les: " + m, file=sys.stderr, flush=True))
-    cl, nc, cs = r["runs"]["closure"], r["runs"]["no_closure"], r["runs"]["closure_per_step"]
-    # the value is the LES as it is meant to run: the network evaluated once per step and its term
-    # held over the three SSPRK stages (same accuracy as evaluating it in every stage, measured
-    # below, at about half the wall-clock)
-    out = {
-        "metric": "LES wall-clock speedup over DNS per unit simulated time (512x257 closure LES vs 4096x2049 DNS)",
-        "value": r["speedup_vs_dns_per_step_closure"], "unit": "x", "n_gpus": 1, "higher_is_better": True,
-        "vs_baseline": None,
-        "dtype": "f64 solver, bf16/fp32 CNN", "data": "synthetic (DNS generated in the run)",
-        "config": {"workload": "learned-closure LES 512x257 (tanh beta=2, Re_b=2e4, F0=0.05 n=12, drag 1e-3) "
-                               "trained on 4096x2049 DNS filtered 8x in x (sharp spectral) and 8x in mapped y (box)",
-                   "pairs": a.les_pairs, "batch": 32, "train_steps": a.les_train_steps,
-                   "spinup_time": a.les_spinup, "window_time": a.les_window,
-                   "closure": "evaluated once per step, held over the three stages"},
-        "re_tau_err": cs.get("re_tau_rel_err"), "re_tau_err_no_closure": nc.get("re_tau_rel_err"),
-        "speedup_closure_every_stage": r["speedup_vs_dns"], "re_tau_err_closure_every_stage": cl.get("re_tau_rel_err"),
-        "U_err_closure": cl.get("U_rel_err"), "U_err_no_closure": nc.get("U_rel_err"),
-        "traj_err_closure": cl.get("traj_err_end"), "traj_err_no_closure": nc.get("traj_err_end"),
-        "details": r,
-    }
+               n_samples=min(100, a.les_pairs), log=lambda m: print("les: " + m, file=sys.stderr, flush=True),
+               compare_time=a.les_compare)
+    rec = les_record(r, a)
+    out = {"metric": rec["metric"], "value": rec["speedup"], "unit": "x", "n_gpus": 1, "higher_is_better": True,
+           "vs_baseline": None, "dtype": "f64 solver, bf16/fp32 CNN", "data": "synthetic (DNS generated in the run)",
+           "config": rec["config"], "les": rec, "details": r}
     print(json.dumps(out), flush=True)
+
+
+def les_leg(a):
+    """The LES leg of the default line, run as a child process (its own GPU context, a time limit)
+    so that whatever happens there the channel figures are still printed: the child's record, or
+    {"error": ...}."""
+    import subprocess
+    cmd = [sys.executable, "-u", os.path.abspath(__file__), "--workload", "les",
+           "--les-spinup", repr(a.les_spinup), "--les-window", repr(a.les_window),
+           "--les-compare", repr(a.les_compare), "--les-pairs", str(a.les_pairs),
+           "--les-train-steps", str(a.les_train_steps)]
+    try:
+        p = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=None, timeout=a.les_timeout, text=True)
+    except subprocess.TimeoutExpired:
+        return {"error": f"LES leg exceeded {a.les_timeout:.0f} s"}
+    if p.returncode != 0:
+        return {"error": f"LES leg exited with status {p.returncode}"}
+    try:
+        return json.loads(p.stdout.strip().splitlines()[-1])["les"]
+    except (ValueError, KeyError, IndexError) as e:
+        return {"error": f"LES leg printed no record ({e})"}
 
 
 def run_cnn(a):
@@ -369,6 +412,10 @@ def main():
                          "network alone; diag: one fused diagnostics row")
     ap.add_argument("--les-spinup", type=float, default=4.0)
     ap.add_argument("--les-window", type=float, default=4.0)
+    ap.add_argument("--les-compare", type=float, default=12.0,
+                    help="LES: length of the comparison window from the start of the training window")
+    ap.add_argument("--les-timeout", type=float, default=600.0, help="channel: time limit of the LES leg")
+    ap.add_argument("--no-les", action="store_true", help="channel: leave out the LES leg")
     ap.add_argument("--les-pairs", type=int, default=2000)
     ap.add_argument("--les-train-steps", type=int, default=1250)
     ap.add_argument("--gpus", type=int, default=1)
@@ -478,6 +525,10 @@ def main():
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(c, a.cpu_seconds)
     L.nse2d_channel_destroy(ch)
+    for buf in keep.values():
+        buf.free()
+    if not a.no_les:
+        out["les"] = les_leg(a)
     print(json.dumps(out), flush=True)
 
 

@@ -438,6 +438,15 @@ int nse2d_les_closure(nse2d_les_t L, const double* d_omega_c, const double* d_ps
  * The channel must have been created on the stream given to nse2d_les_create (else -1).
  * nse2d_les_closure always evaluates, whatever is attached. 0 on success. */
 int nse2d_les_attach(nse2d_les_t L, nse2d_channel_t coarse_ch, int on);
+/* Constraints applied to every later closure evaluation (attached or nse2d_les_closure), as an
+ * OR of the flags below; 0 (the default) applies none. In the order they are applied:
+ * NSE2D_CLOSURE_MEAN_FREE subtracts each row's x-average from the term, leaving the mean
+ * vorticity profile to the resolved dynamics; NSE2D_CLOSURE_DISSIPATIVE sets the term to 0 where
+ * omega * C > 0 (backscatter, which adds enstrophy). No counterpart in the reference, whose
+ * solver has no closure: these are options of the LES built on it. 0, or -1 on an unknown flag. */
+#define NSE2D_CLOSURE_DISSIPATIVE 1
+#define NSE2D_CLOSURE_MEAN_FREE 2
+int nse2d_les_set_constraints(nse2d_les_t L, int flags);
 
 /* ------------------------------------------------------------------------------------------
  * 6. Runtime helpers (device memory, streams) so that callers without another HIP binding

@@ -215,6 +215,7 @@ struct LesImpl {
     nse2d_cnn_t cnn = nullptr;
     int attach_mode = 1;   // 1: closure of every stage state; 2: of the step's first stage, held
     const double* attached_w = nullptr;  // omega^n of the attached channel (its first stage state)
+    int constraints = 0;  // NSE2D_CLOSURE_* flags applied to every closure evaluation
 };
 
 namespace {
@@ -264,12 +265,46 @@ Norm raw_norm() {
 
 // The closure term of the coarse state (w, psi) into dst, on L's stream: the inputs go straight
 // into the network's packed input, the scaled term straight into dst.
+// Mean-free closure: the x-average of C over each row is subtracted, so the term leaves the
+// mean vorticity profile (and with it the mean velocity and the wall shear) to the resolved
+// dynamics. One workgroup of 256 per row; the row sum is taken in double.
+__global__ void __launch_bounds__(256) mean_free_rows_kernel(float* __restrict__ C, int Nx) {
+    __shared__ double part[256];
+    float* row = C + (size_t)blockIdx.x * Nx;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < Nx; i += 256) acc += (double)row[i];
+    part[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    const float mean = (float)(part[0] / Nx);
+    for (int i = threadIdx.x; i < Nx; i += 256) row[i] -= mean;
+}
+
+// Backscatter removal: where omega * C > 0 the closure term would add enstrophy (d(omega^2/2)/dt
+// gains omega * C); those points get C = 0, so the closure only removes enstrophy.
+__global__ void dissipative_clip_kernel(float* __restrict__ C, const double* __restrict__ w, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && (double)C[i] * w[i] > 0.0) C[i] = 0.0f;
+}
+
 void closure_eval(LesImpl* L, const double* w, const double* psi, float* dst) {
     const Grid& g = L->c;
     features_kernel<true><<<dim3(ceil_div(g.d.Nx, 256), g.d.Ny), 256, 0, L->s>>>(
         w, psi, g.d.Nx, g.d.Ny, g.inv_dx, g.inv_deta, g.d.d_a_node, cnn_packed_input(L->cnn), L->norm);
     NSE2D_LAUNCH_CHECK();
     cnn_infer_packed_closure(L->cnn, (float)L->stdv[kFeat], (float)L->mean[kFeat], dst);
+    if (L->constraints & NSE2D_CLOSURE_MEAN_FREE) {
+        mean_free_rows_kernel<<<g.d.Ny, 256, 0, L->s>>>(dst, g.d.Nx);
+        NSE2D_LAUNCH_CHECK();
+    }
+    if (L->constraints & NSE2D_CLOSURE_DISSIPATIVE) {
+        const size_t n = (size_t)g.d.Nx * g.d.Ny;
+        dissipative_clip_kernel<<<(unsigned)((n + 255) / 256), 256, 0, L->s>>>(dst, w, n);
+        NSE2D_LAUNCH_CHECK();
+    }
 }
 
 // The closure hook of a coarse channel (nse2d_closure_fn). The channel runs on L's stream
@@ -515,6 +550,14 @@ int nse2d_les_eval(nse2d_les_t L, int first, int count, double* loss_cnn, double
 }
 int nse2d_les_closure(nse2d_les_t L, const double* d_omega_c, const double* d_psi_c, float* d_out) {
     closure_eval(L->impl, d_omega_c, d_psi_c, d_out);
+    return 0;
+}
+int nse2d_les_set_constraints(nse2d_les_t L, int flags) {
+    if (flags & ~(NSE2D_CLOSURE_DISSIPATIVE | NSE2D_CLOSURE_MEAN_FREE)) {
+        set_last_error("nse2d_les_set_constraints: unknown flag");
+        return -1;
+    }
+    L->impl->constraints = flags;
     return 0;
 }
 int nse2d_les_attach(nse2d_les_t L, nse2d_channel_t coarse_ch, int on) {

@@ -5,9 +5,11 @@
      (nse2d_les_pair: box filter in mapped y, sharp spectral cut in x, FP64);
   3. Adam training of the CNN on the device (nse2d_les_train, bf16 MFMA kernels);
   4. coarse runs from the filtered DNS state, with the closure attached (nse2d_les_attach: in
-     every SSPRK stage, or once per step) and without it, whose mean profile U(y) and Re_tau are
-     compared with those of the DNS over the same time window, and whose wall-clock time per unit
-     of simulated time is compared with the DNS's.
+     every SSPRK stage, or once per step) and without it, whose mean profile U(y), Re_tau and
+     energy/enstrophy budget (K, Omega, the enstrophy dissipation eta and the forcing's enstrophy
+     input P_Omega, the reductions of main.cu:627-629 through nse2d_diag_row) are compared with
+     those of the filtered DNS over the training window and, optionally, a longer comparison
+     window, and whose wall-clock time per unit of simulated time is compared with the DNS's.
 
 No reference counterpart (the reference runs DNS only). Nothing here runs on the CPU beyond the
 bookkeeping of a few profiles; there is no fallback.
@@ -202,8 +204,35 @@ def _interior_rel(w, ref):
     return float(np.linalg.norm(d) / np.linalg.norm(ref[1:-1]))
 
 
+class Budget:
+    """K, Omega, eta and P_Omega of a state on a channel's grid (main.cu:627-629: the raw sums of
+    nse2d_diag_row times 1/A, with 1/2, nu as there)."""
+
+    def __init__(self, ch):
+        self.ch = ch
+        self.out = nh.DeviceBuffer(nh.DIAG_NOUT)
+        self.invA = 1.0 / (ch.Nx * float(np.sum(ch.m["w_node"])))
+
+    def __call__(self, w_ptr, psi_ptr):
+        ch, c = self.ch, self.ch.c
+        # <omega rhs> is not used here: omega stands in for the rhs argument
+        rc = nh.lib().nse2d_diag_row(psi_ptr, w_ptr, w_ptr, ch.Nx, ch.Ny, ch.Nx / c["Lx"], (ch.Ny - 1) / 2.0, c["h"],
+                                     c["F0"], c["nforce"], ch.dm["a_node"].ptr, ch.dm["a_edge"].ptr, ch.dm["y"].ptr,
+                                     ch.dm["w_node"].ptr, self.out.ptr, None, None)
+        if rc:
+            raise RuntimeError(nh.last_error())
+        o = self.out.download()
+        return {"K": o[nh.DIAG_K] * self.invA, "Omega": 0.5 * o[nh.DIAG_OMEGA_W2] * self.invA,
+                "eta": ch.nu * o[nh.DIAG_OMEGA_GRAD2] * self.invA, "P_Omega": o[nh.DIAG_POMEGA] * self.invA}
+
+
+def _budget_errs(acc, ref):
+    """Relative errors of the time-averaged budget terms."""
+    return {k: abs(float(np.mean([a[k] for a in acc])) - ref[k]) / abs(ref[k]) for k in ref}
+
+
 def run(fine_cfg, coarse_cfg, *, spinup_time, window_time, n_pairs, holdout, train_steps, batch, lr, n_samples,
-        log=print, seed=1):
+        log=print, seed=1, compare_time=None):
     """The whole experiment; returns a dict of results.
 
     The DNS window [t0, t1] yields n_pairs training pairs at equal time spacing; n_samples of them
@@ -214,12 +243,17 @@ def run(fine_cfg, coarse_cfg, *, spinup_time, window_time, n_pairs, holdout, tra
     measured with the coarse grid's own wall estimator (Thom closure of the filtered state, the
     estimator an LES has), and also reported for the fine grid. Finally each coarse run is timed
     over the window with the device dt (no host round trip) for the speed comparison with the DNS.
+    With compare_time > window_time the DNS goes on past the training window and the comparison
+    (not the training) covers [t0, t0 + compare_time], at the same sample spacing; the statistics
+    are also reported over the training window alone ("window" entries).
     """
     L = nh.lib()
     fine = Channel(fine_cfg)
     coarse_cl = Channel(coarse_cfg, ic=False)
     coarse_nc = Channel(coarse_cfg, ic=False)
     les = Les(fine, coarse_cl, n_pairs, batch, seed)
+    budget = Budget(coarse_cl)
+    compare_time = max(window_time, compare_time or window_time)
     out = {"fine": [fine.Nx, fine.Ny], "coarse": [coarse_cl.Nx, coarse_cl.Ny], "n_pairs": n_pairs,
            "holdout": holdout, "train_steps": train_steps, "batch": batch}
     nu_c = coarse_cl.nu
@@ -256,9 +290,24 @@ def run(fine_cfg, coarse_cfg, *, spinup_time, window_time, n_pairs, holdout, tra
             if k % every == 0:
                 wc = les.wc.download((coarse_cl.Ny, coarse_cl.Nx))
                 re_c = math.sqrt(0.5 * nu_c * (abs(wc[0].mean()) + abs(wc[-1].mean()))) * coarse_cl.c["h"] / nu_c
-                samples.append((fine.time(), wc, coarse_cl.mean_u(les.psic.ptr), fine.re_tau(), re_c))
+                samples.append((fine.time(), wc, coarse_cl.mean_u(les.psic.ptr), fine.re_tau(), re_c,
+                                budget(les.wc.ptr, les.psic.ptr)))
             k += 1
         t1 = fine.time()
+        n_window = len(samples)
+        # 2b. filtered samples past the training window (comparison only, no pairs kept)
+        dt_sample = every * dt_pair
+        while samples[-1][0] + dt_sample <= t0 + compare_time + 1e-9:
+            target_t = samples[-1][0] + dt_sample
+            while fine.time() < target_t:
+                fine.step(10)
+            les.pair(append=False)
+            wc = les.wc.download((coarse_cl.Ny, coarse_cl.Nx))
+            re_c = math.sqrt(0.5 * nu_c * (abs(wc[0].mean()) + abs(wc[-1].mean()))) * coarse_cl.c["h"] / nu_c
+            samples.append((fine.time(), wc, coarse_cl.mean_u(les.psic.ptr), fine.re_tau(), re_c,
+                            budget(les.wc.ptr, les.psic.ptr)))
+        t_end = samples[-1][0]
+        out["compare_window"] = [t0, t_end]
         out["dns_window"] = [t0, t1]
         out["dns_steps_window"] = dns_steps
         out["dns_wall_per_time"] = dns_wall / max(t1 - t0, 1e-30)
@@ -276,20 +325,31 @@ def run(fine_cfg, coarse_cfg, *, spinup_time, window_time, n_pairs, holdout, tra
             f"{losses[:10].mean():.4f} -> {losses[-10:].mean():.4f}; hold-out normalised MSE {lc:.4f} "
             f"(zero closure {lz:.4f})")
         # 4. coarse runs from the first filtered state, compared at the sample times
-        U_dns = np.mean([s[2] for s in samples[1:]], axis=0)
-        re_fine = float(np.mean([s[3] for s in samples[1:]]))
-        re_filt = float(np.mean([s[4] for s in samples[1:]]))
+        def dns_stats(sl):
+            ss = samples[sl]
+            return (np.mean([s[2] for s in ss], axis=0), float(np.mean([s[3] for s in ss])),
+                    float(np.mean([s[4] for s in ss])),
+                    {k: float(np.mean([s[5][k] for s in ss])) for k in ss[0][5]})
+        U_dns, re_fine, re_filt, bud_dns = dns_stats(slice(1, None))
+        U_dns_w, re_fine_w, re_filt_w, bud_dns_w = dns_stats(slice(1, n_window))
         out["re_tau_dns_fine"], out["re_tau_dns_filtered"] = re_fine, re_filt
+        out["budget_dns_filtered"] = bud_dns
         res = {}
-        for name, ch, mode in (("closure", coarse_cl, 1), ("closure_per_step", coarse_cl, 2),
-                               ("no_closure", coarse_nc, 0)):
+        mf, dis = nh.CLOSURE_MEAN_FREE, nh.CLOSURE_DISSIPATIVE
+        for name, ch, mode, flags in (("closure", coarse_cl, 1, 0), ("closure_per_step", coarse_cl, 2, 0),
+                                      ("closure_mean_free", coarse_cl, 2, mf),
+                                      ("closure_dissipative", coarse_cl, 2, dis),
+                                      ("closure_mean_free_dissipative", coarse_cl, 2, mf | dis),
+                                      ("no_closure", coarse_nc, 0, 0)):
+            if nh.lib().nse2d_les_set_constraints(les.h, flags) != 0:
+                raise RuntimeError(nh.last_error())
             if mode:
                 les.attach(ch, mode)
             ch.set_omega(samples[0][1], t0)
             t = t0
-            errs, U_acc, re_acc = [], [], []
+            errs, U_acc, re_acc, b_acc = [], [], [], []
             ok = True
-            for (tk, wk, _, _, _) in samples[1:]:
+            for (tk, wk, _, _, _, _) in samples[1:]:
                 t = ch.advance_to(t, tk)
                 w = ch.omega()
                 if not np.all(np.isfinite(w)):
@@ -298,12 +358,27 @@ def run(fine_cfg, coarse_cfg, *, spinup_time, window_time, n_pairs, holdout, tra
                 errs.append(_interior_rel(w, wk))
                 U_acc.append(ch.mean_u())
                 re_acc.append(ch.re_tau())
+                b_acc.append(budget(ch.field(nh.FIELD_OMEGA), ch.field(nh.FIELD_PSI)))
+            nw = n_window - 1  # samples of the training window after the start
             r = {"finite": ok, "traj_err_end": errs[-1] if errs else None,
-                 "traj_err_mean": float(np.mean(errs)) if errs else None}
+                 "traj_err_mean": float(np.mean(errs)) if errs else None,
+                 "traj_err_window_end": errs[nw - 1] if len(errs) >= nw else None}
             if ok and U_acc:
                 Um, rl = np.mean(U_acc, axis=0), float(np.mean(re_acc))
                 r.update(re_tau=rl, re_tau_rel_err=abs(rl - re_filt) / re_filt,
-                         re_tau_rel_err_vs_fine=abs(rl - re_fine) / re_fine, U_rel_err=_rel_profile_err(Um, U_dns))
+                         re_tau_rel_err_vs_fine=abs(rl - re_fine) / re_fine, U_rel_err=_rel_profile_err(Um, U_dns),
+                         budget=_budget_errs(b_acc, bud_dns))
+                Uw, rw = np.mean(U_acc[:nw], axis=0), float(np.mean(re_acc[:nw]))
+                r["window"] = {"re_tau_rel_err": abs(rw - re_filt_w) / re_filt_w,
+                               "U_rel_err": _rel_profile_err(Uw, U_dns_w),
+                               "budget": _budget_errs(b_acc[:nw], bud_dns_w)}
+                # Re_tau over sliding thirds of the comparison window (does the error grow?)
+                n3 = max(1, len(re_acc) // 3)
+                r["re_tau_rel_err_thirds"] = [
+                    abs(float(np.mean(re_acc[i * n3:(i + 1) * n3 if i < 2 else None])) -
+                        float(np.mean([s[4] for s in samples[1:][i * n3:(i + 1) * n3 if i < 2 else None]]))) /
+                    float(np.mean([s[4] for s in samples[1:][i * n3:(i + 1) * n3 if i < 2 else None]]))
+                    for i in range(3)]
             # timing over the window with the device dt
             ch.set_omega(samples[0][1], t0)
             L.nse2d_device_sync()
@@ -317,8 +392,8 @@ def run(fine_cfg, coarse_cfg, *, spinup_time, window_time, n_pairs, holdout, tra
             r.update(timed_steps=steps, wall_per_time=wall / max(ch.time() - t0, 1e-30))
             res[name] = r
             log(f"{name}: trajectory error {r['traj_err_end']}, Re_tau {r.get('re_tau')} vs filtered DNS "
-                f"{re_filt:.2f} / fine {re_fine:.2f}, U(y) error {r.get('U_rel_err')}, "
-                f"{r['wall_per_time']:.3f} s per unit time")
+                f"{re_filt:.2f} / fine {re_fine:.2f}, U(y) error {r.get('U_rel_err')}, budget errors "
+                f"{r.get('budget')}, {r['wall_per_time']:.3f} s per unit time")
         out["runs"] = res
         out["speedup_vs_dns"] = out["dns_wall_per_time"] / res["closure"]["wall_per_time"]
         out["speedup_vs_dns_per_step_closure"] = out["dns_wall_per_time"] / res["closure_per_step"]["wall_per_time"]

@@ -115,6 +115,7 @@ SIGNATURES = {
     "nse2d_les_train": (_d, [_vp, _i, _i, _i, ctypes.c_float, ctypes.c_ulonglong, ctypes.POINTER(ctypes.c_float)]),
     "nse2d_les_eval": (_i, [_vp, _i, _i, _p, _p]),
     "nse2d_les_closure": (_i, [_vp, _vp, _vp, _vp]),
+    "nse2d_les_set_constraints": (_i, [_vp, _i]),
     "nse2d_les_attach": (_i, [_vp, _vp, _i]),
     # ensembles
     "nse2d_ensemble_create": (_vp, [_vp, _i, _stream]),
@@ -149,6 +150,11 @@ SIGNATURES = {
 FIELD_OMEGA, FIELD_PSI, FIELD_RHS, FIELD_W1, FIELD_W2 = 0, 1, 2, 3, 4
 PROF_CLASSES = ["stage", "fft_forward", "thomas", "fft_inverse", "thom", "cfl"]
 STEP_STORE_RHS, STEP_WALL_BOOKKEEPING = 1, 2
+CLOSURE_DISSIPATIVE, CLOSURE_MEAN_FREE = 1, 2
+# outputs of nse2d_diag_row (NSE2D_DIAG_* of include/nse2d_hip.h)
+(DIAG_UMAX, DIAG_VMAX, DIAG_VMAXM, DIAG_K, DIAG_OMEGA_W2, DIAG_OMEGA_GRAD2, DIAG_OMEGA_RHS, DIAG_PIN, DIAG_EPS,
+ DIAG_POMEGA, DIAG_WALL_OMEGA_OMEGAY_BOT, DIAG_WALL_OMEGA_OMEGAY_TOP, DIAG_WALL_OMEGA_BOT, DIAG_WALL_OMEGA_TOP,
+ DIAG_NOUT) = range(15)
 
 
 class ChannelDesc(ctypes.Structure):

@@ -239,6 +239,49 @@ def test_closure_term_matches_the_cnn_oracle():
     assert e <= 5e-3
 
 
+def test_closure_constraints():
+    """nse2d_les_set_constraints: the mean-free term is the plain term minus its row averages, the
+    dissipative one is the plain term with every point where omega * C > 0 set to 0, both together
+    apply the row averages first; an unknown flag is refused."""
+    F = lx.Channel(cfg(256, 129), ic=False)
+    C = lx.Channel(cfg(128, 65), ic=False)
+    les = lx.Les(F, C, 1, 1, seed=5)
+    g = Grid(128, 65, 2.0)
+    w, psi = fine_state(g, 2.0, 9)
+    d_w, d_psi = nh.DeviceBuffer.from_array(w), nh.DeviceBuffer.from_array(psi)
+    d_out = nh.DeviceArray(shape=(65, 128), dtype=np.float32)
+    lib = nh.lib()
+
+    def term(flags):
+        assert lib.nse2d_les_set_constraints(les.h, flags) == 0
+        assert lib.nse2d_les_closure(les.h, d_w.ptr, d_psi.ptr, d_out.ptr) == 0
+        return d_out.download()
+
+    base = term(0)
+    mf = term(nh.CLOSURE_MEAN_FREE)
+    dis = term(nh.CLOSURE_DISSIPATIVE)
+    both = term(nh.CLOSURE_MEAN_FREE | nh.CLOSURE_DISSIPATIVE)
+    bad = lib.nse2d_les_set_constraints(les.h, 4)
+    again = term(0)
+    les.destroy()
+    F.destroy()
+    C.destroy()
+    means = base.astype(np.float64).mean(axis=1, keepdims=True)
+    mf_ref = base - means.astype(np.float32)
+    scale = float(np.max(np.abs(base)))
+    e_mf = float(np.max(np.abs(mf - mf_ref))) / scale
+    rows = float(np.max(np.abs(mf.astype(np.float64).sum(axis=1)) / np.abs(base).astype(np.float64).sum(axis=1).clip(1e-300)))
+    print(f"[les] mean-free term {e_mf:.1e}, row sums {rows:.1e}; backscatter points "
+          f"{int(np.sum(base * w > 0))} of {base.size}", flush=True)
+    assert np.std(means) > 0.0  # the plain term has row means to remove
+    assert e_mf <= 1e-6 and rows <= 1e-5
+    assert np.array_equal(dis, np.where(base.astype(np.float64) * w > 0.0, np.float32(0.0), base))
+    assert 0 < np.sum(base * w > 0) < base.size
+    assert np.array_equal(both, np.where(mf.astype(np.float64) * w > 0.0, np.float32(0.0), mf))
+    assert bad == -1 and "unknown flag" in nh.last_error()
+    assert np.array_equal(again, base)
+
+
 @pytest.mark.parametrize("fine,coarse", [((128, 65), (64, 33)), ((4096, 2049), (512, 257))],
                          ids=["128x65-64x33", "4096x2049-512x257"])
 def test_once_per_step_closure_with_direct_calls_between(fine, coarse, progress):
