@@ -245,11 +245,21 @@ def les_record(r, a):
             "within_2pct": bool(x["finite"] and (x.get("re_tau_rel_err") or 1.0) <= 0.02
                                 and (x.get("U_rel_err") or 1.0) <= 0.02)}
     t0 = a.les_spinup
+    dns = r["dns_wall_per_time"]
+    for x in runs.values():
+        x["speedup"] = dns / x["wall_s_per_unit_time"] if x["wall_s_per_unit_time"] > 0 else None
+    # the metric's figure: the fastest closure run whose Re_tau and U(y) stay within 2 % of the
+    # filtered DNS over the whole comparison window (null when none does)
+    ok = [n for n in ("closure_per_step", "closure_mean_free") if runs[n]["within_2pct"] and runs[n]["speedup"]]
+    best = max(ok, key=lambda n: runs[n]["speedup"]) if ok else None
     return {
-        "metric": "LES wall-clock speedup over DNS per unit simulated time (512x257 closure LES vs 4096x2049 DNS)",
-        "speedup": r["speedup_vs_dns_per_step_closure"], "unit": "x",
+        "metric": "LES wall-clock speedup over DNS per unit simulated time at <=2% Re_tau and U(y) error "
+                  "(512x257 closure LES vs 4096x2049 DNS)",
+        "speedup": runs[best]["speedup"] if best else None, "unit": "x", "speedup_run": best,
+        "speedup_closure_per_step_unconstrained": r["speedup_vs_dns_per_step_closure"],
         "speedup_closure_every_stage": r["speedup_vs_dns"],
-        "dns_wall_s_per_unit_time": r["dns_wall_per_time"],
+        "speedup_no_closure": runs["no_closure"]["speedup"],
+        "dns_wall_s_per_unit_time": dns,
         "config": {"workload": LES_CONFIG, "pairs": a.les_pairs, "batch": 32, "train_steps": a.les_train_steps,
                    "training_window": [t0, t0 + a.les_window], "comparison_window": [t0, t0 + a.les_compare],
                    "closure": "evaluated once per step, held over the three stages"},
