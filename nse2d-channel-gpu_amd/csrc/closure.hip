@@ -465,7 +465,7 @@ __device__ inline bf16x8 window8(const bf16x8& c0, const bf16x8& c1) {
 // The bias gradient (column sums of dZ) is summed from the staged dZ values on the way; the block's
 // 64 sums go to bias_partial[blockIdx.x].
 template <int CIN>
-__global__ void __launch_bounds__(CIN * 4) conv_wgrad_kernel(const bf16* __restrict__ x, const bf16* __restrict__ dz,
+__global__ void __launch_bounds__(CIN * 4) __attribute__((amdgpu_waves_per_eu(CIN == 64 ? 2 : 1, 2))) conv_wgrad_kernel(const bf16* __restrict__ x, const bf16* __restrict__ dz,
                                                              float* __restrict__ partial,
                                                              float* __restrict__ bias_partial, int H, int W, int B,
                                                              int segs_per_block) {
@@ -480,46 +480,84 @@ __global__ void __launch_bounds__(CIN * 4) conv_wgrad_kernel(const bf16* __restr
     floatx16 acc[9];
 #pragma unroll
     for (int t = 0; t < 9; ++t) acc[t] = floatx16{0};
-    static_assert(NT % (kC / 8) == 0, "each thread stages the same eight channels of dZ");
-    float bsum[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // channels (tid % 8) * 8 + q
     const int nseg = W >> 6, total = B * H * nseg;
     const int s0 = blockIdx.x * segs_per_block, s1 = min(total, s0 + segs_per_block);
-    for (int sg = s0; sg < s1; ++sg) {
+    // Segment sg's global operands: a pixel pair x eight channels of dZ per element e < 32 (kC/8)
+    // and of the three input rows per element e < 3 * 33 (CIN/8), the pixel pair fastest across
+    // lanes: the transposed 4-byte LDS writes of a wavefront then fill consecutive words of one
+    // channel row (with the channel octet fastest they fell 8 rows apart, on the same two banks).
+    // They are fetched into registers one segment ahead: the next segment's loads are issued
+    // before the current segment's products, so their latency runs under the MFMAs.
+    constexpr int NDZ = (32 * (kC / 8) + NT - 1) / NT, NX = (3 * 33 * (CIN / 8) + NT - 1) / NT;
+    static_assert((32 * (kC / 8)) % NT == 0, "every thread stages NDZ elements of dZ");
+    bf16x8 pdz[NDZ][2], px[NX][2];
+    float bsum[NDZ][8];  // element e = tid + n NT: channels (e / 32) * 8 + q
+#pragma unroll
+    for (int n = 0; n < NDZ; ++n)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) bsum[n][q] = 0.0f;
+    auto fetch = [&](int sg) {
         const int seg = sg % nseg, y = (sg / nseg) % H, b = sg / (nseg * H);
         const int x0 = seg << 6;
-        // stage dZ^T: pixel pairs (2m, 2m+1) x 8 channels
-        for (int e = tid; e < 32 * (kC / 8); e += NT) {
-            const int m = e / (kC / 8), c8 = (e % (kC / 8)) * 8;
-            const bf16* src = dz + (((size_t)b * H + y) * W + x0 + 2 * m) * kC + c8;
-            const bf16x8 v0 = load8(src), v1 = load8(src + kC);
 #pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                *reinterpret_cast<unsigned*>(&dzT[c8 + q][2 * m]) = pack_bf16x2(v0[q], v1[q]);
-                bsum[q] += (float)v0[q] + (float)v1[q];
+        for (int n = 0; n < NDZ; ++n) {
+            const int e = tid + n * NT;
+            if (e < 32 * (kC / 8)) {
+                const int m = e % 32, c8 = (e / 32) * 8;
+                const bf16* src = dz + (((size_t)b * H + y) * W + x0 + 2 * m) * kC + c8;
+                pdz[n][0] = load8(src);
+                pdz[n][1] = load8(src + kC);
             }
         }
-        // stage the input rows: pixel pairs (j, j+1) = (2m, 2m+1), m = 0..32, of rows y-1..y+1
-        for (int e = tid; e < 3 * 33 * (CIN / 8); e += NT) {
+        // unconditional loads (element index and row clamped into range; rows beyond the walls
+        // are zeroed when staged): under a branch, the zero fill of the operand registers made
+        // the compiler wait for every load issued before it, one memory latency per element
+#pragma unroll
+        for (int n = 0; n < NX; ++n) {
+            const int e = min(tid + n * NT, 3 * 33 * (CIN / 8) - 1);
             const int dyi = e / (33 * (CIN / 8)), rem = e % (33 * (CIN / 8));
-            const int m = rem / (CIN / 8), c8 = (rem % (CIN / 8)) * 8;
-            const int yy = y + dyi - 1;
-            bf16x8 v0, v1;
-            if (yy >= 0 && yy < H) {
-                int xa = x0 - 1 + 2 * m, xb = x0 + 2 * m;
-                xa = xa < 0 ? xa + W : xa;
-                xb = xb >= W ? xb - W : xb;
-                const bf16* row = x + ((size_t)b * H + yy) * W * CIN + c8;
-                v0 = load8(row + (size_t)xa * CIN);
-                v1 = load8(row + (size_t)xb * CIN);
-            } else {
+            const int m = rem % 33, c8 = (rem / 33) * 8;
+            const int yy = min(max(y + dyi - 1, 0), H - 1);
+            int xa = x0 - 1 + 2 * m, xb = x0 + 2 * m;
+            xa = xa < 0 ? xa + W : xa;
+            xb = xb >= W ? xb - W : xb;
+            const bf16* row = x + ((size_t)b * H + yy) * W * CIN + c8;
+            px[n][0] = load8(row + (size_t)xa * CIN);
+            px[n][1] = load8(row + (size_t)xb * CIN);
+        }
+    };
+    if (s0 < s1) fetch(s0);
+    for (int sg = s0; sg < s1; ++sg) {
+        // stage dZ^T (pixel pairs (2m, 2m+1) x 8 channels) and the input rows (pixel pairs
+        // (j, j+1) = (2m, 2m+1), m = 0..32, of rows y-1..y+1), transposed
 #pragma unroll
-                for (int q = 0; q < 8; ++q) v0[q] = v1[q] = (bf16)0.0f;
+        for (int n = 0; n < NDZ; ++n) {
+            const int e = tid + n * NT;
+            if (e < 32 * (kC / 8)) {
+                const int m = e % 32, c8 = (e / 32) * 8;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    *reinterpret_cast<unsigned*>(&dzT[c8 + q][2 * m]) = pack_bf16x2(pdz[n][0][q], pdz[n][1][q]);
+                    bsum[n][q] += (float)pdz[n][0][q] + (float)pdz[n][1][q];
+                }
             }
+        }
+        const int ycur = (sg / nseg) % H;
 #pragma unroll
-            for (int q = 0; q < 8; ++q)
-                *reinterpret_cast<unsigned*>(&xT[dyi][c8 + q][2 * m]) = pack_bf16x2(v0[q], v1[q]);
+        for (int n = 0; n < NX; ++n) {
+            const int e = tid + n * NT;
+            if (e < 3 * 33 * (CIN / 8)) {
+                const int dyi = e / (33 * (CIN / 8)), rem = e % (33 * (CIN / 8));
+                const int m = rem % 33, c8 = (rem / 33) * 8;
+                const int yy = ycur + dyi - 1;
+                const bool in = yy >= 0 && yy < H;
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    *reinterpret_cast<unsigned*>(&xT[dyi][c8 + q][2 * m]) = in ? pack_bf16x2(px[n][0][q], px[n][1][q]) : 0u;
+            }
         }
         __syncthreads();
+        if (sg + 1 < s1) fetch(sg + 1);  // in flight during the products below
 #pragma unroll
         for (int dyi = 0; dyi < 3; ++dyi)
 #pragma unroll
@@ -534,15 +572,19 @@ __global__ void __launch_bounds__(CIN * 4) conv_wgrad_kernel(const bf16* __restr
             }
         __syncthreads();
     }
-    {   // bias: the NT / 8 threads of each channel octet, added in a fixed order
-        static_assert(sizeof(xT) >= sizeof(float) * NT * 8, "bias sums fit in the input tile");
-        float* bs = reinterpret_cast<float*>(&xT[0][0][0]);  // [NT / 8][64], free after the last barrier
+    {   // bias: the 32 pixel-pair lanes of each channel, added in a fixed order
+        static_assert(sizeof(xT) >= sizeof(float) * 32 * kC, "bias sums fit in the input tile");
+        float* bs = reinterpret_cast<float*>(&xT[0][0][0]);  // [32][64], free after the last barrier
 #pragma unroll
-        for (int q = 0; q < 8; ++q) bs[(tid >> 3) * kC + (tid & 7) * 8 + q] = bsum[q];
+        for (int n = 0; n < NDZ; ++n) {
+            const int e = tid + n * NT;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) bs[(e % 32) * kC + (e / 32) * 8 + q] = bsum[n][q];
+        }
         __syncthreads();
         if (tid < kC) {
             float t = 0.0f;
-            for (int r = 0; r < NT / 8; ++r) t += bs[r * kC + tid];
+            for (int r = 0; r < 32; ++r) t += bs[r * kC + tid];
             bias_partial[(size_t)blockIdx.x * kC + tid] = t;
         }
     }
@@ -909,9 +951,16 @@ template <int CIN>
 void wgrad(CnnImpl* c, const cnn::bf16* x, const cnn::bf16* dz, float* gw, float* gb, int B) {
     const int total = B * c->H * (c->W / 64);
     // one round of resident workgroups, each over a long run of row segments (keeps the partial
-    // sums small): at about 200 VGPRs a SIMD holds two wavefronts, so a CU holds 8 / (CIN / 16)
-    // workgroups of CIN / 16 wavefronts (two at CIN = 64, four at CIN = 32)
-    const int nb = std::min(std::min(c->wgrad_blocks, c->cus * (128 / CIN)), total);
+    // sums small): as many workgroups per CU as the kernel's occupancy allows (two of four
+    // wavefronts at CIN = 64; two of two at CIN = 32, whose accumulators and prefetched operands
+    // hold one wavefront per SIMD)
+    static const int per_cu = [] {
+        int n = 0;
+        NSE2D_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &n, reinterpret_cast<const void*>(cnn::conv_wgrad_kernel<CIN>), CIN * 4, 0));
+        return std::max(1, n);
+    }();
+    const int nb = std::min(std::min(c->wgrad_blocks, c->cus * per_cu), total);
     const int per = (total + nb - 1) / nb;
     const int used = (total + per - 1) / per;
     float* bias_partial = c->partial + (size_t)c->wgrad_blocks * 9 * cnn::kC * cnn::kC;
