@@ -45,22 +45,24 @@ constexpr int kCin0 = 32;   // padded input channels of the first layer
 constexpr int kInputs = 4;  // real input channels
 constexpr int kLayers = 6;
 
-// tanh(u) = 1 - 2 / (1 + e^{2u}) on the native exponential and reciprocal (v_exp_f32, v_rcp_f32,
-// 1 ulp each): absolute error ~1e-7, far below the bf16 resolution of the activations it feeds
-// (the GELU below adds it to 1), and a few instructions instead of the library tanhf and an IEEE
-// division (the epilogue of every convolution applies it to each output). Saturates correctly:
-// e^{2u} -> inf gives 1, -> 0 gives -1.
-__device__ inline float tanh_fast(float u) { return 1.0f - 2.0f * __builtin_amdgcn_rcpf(1.0f + __expf(2.0f * u)); }
-// GELU, tanh form (Hendrycks and Gimpel): 0.5 z (1 + tanh(s (z + 0.044715 z^3))), s = sqrt(2/pi)
-__device__ inline float gelu(float z) {
-    const float s = 0.7978845608028654f;
-    return 0.5f * z * (1.0f + tanh_fast(s * (z + 0.044715f * z * z * z)));
+// GELU, tanh form (Hendrycks and Gimpel): 0.5 z (1 + tanh(u)), u = s (z + 0.044715 z^3),
+// s = sqrt(2/pi), evaluated as z sigma(2u) with sigma(2u) = 1 / (1 + 2^(k (z + 0.044715 z^3))),
+// k = -2 s log2(e) (0.5 (1 + tanh u) = sigma(2u)): the native base-2 exponential and reciprocal
+// (v_exp_f32, v_rcp_f32, 1 ulp each), about 1e-7 relative, far below the bf16 resolution of the
+// values it feeds, and fewer instructions than the tanh form (the epilogue of every convolution
+// applies it to each output). Saturates correctly: the exponential -> inf gives 0, -> 0 gives z.
+constexpr float kGeluS = 0.7978845608028654f, kGeluC = 0.044715f;
+constexpr float kGeluK = -2.0f * 0.7978845608028654f * 1.4426950408889634f;
+__device__ inline float gelu_sigma(float z) {
+    const float p = __builtin_fmaf(kGeluC * z * z, z, z);
+    return __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(kGeluK * p));
 }
+__device__ inline float gelu(float z) { return z * gelu_sigma(z); }
+// d/dz z sigma(2u) = sigma + 2 z sigma (1 - sigma) u',  u' = s (1 + 3 * 0.044715 z^2)
 __device__ inline float gelu_grad(float z) {
-    const float s = 0.7978845608028654f;
-    const float u = s * (z + 0.044715f * z * z * z);
-    const float th = tanh_fast(u);
-    return 0.5f * (1.0f + th) + 0.5f * z * (1.0f - th * th) * s * (1.0f + 3.0f * 0.044715f * z * z);
+    const float sg = gelu_sigma(z);
+    const float du2 = __builtin_fmaf(2.0f * kGeluS * 3.0f * kGeluC, z * z, 2.0f * kGeluS);
+    return __builtin_fmaf(z * sg * (1.0f - sg), du2, sg);
 }
 
 __device__ inline bf16x8 load8(const bf16* p) { return *reinterpret_cast<const bf16x8*>(p); }
@@ -235,8 +237,18 @@ __global__ void __launch_bounds__(256) conv_lds_kernel(const bf16* __restrict__ 
 template <int CIN>
 struct ConvLdsAll {
     static constexpr int CP = CIN + 8;
+    // The tile's outputs (z and a) are staged in the input tile's space once the products are
+    // done, [row][pixel][64 channels] with the 16-byte chunk c of pixel p at chunk position
+    // c ^ (p & 7), and then stored with 16 contiguous bytes per lane (the accumulator layout gives
+    // a lane four channels of a pixel, so direct stores wrote 8 bytes per lane, 128 bytes apart).
+    // NOUT buffers: two at CIN = 64; one at CIN = 32, where two workgroups per CU leave room for
+    // one, and z and a go out in two rounds.
+    static constexpr int NOUT = CIN == kC ? 2 : 1;
     bf16 w[9][kC][CP];
-    bf16 in[kTileRows + 2][66][CP];
+    union {
+        bf16 in[kTileRows + 2][66][CP];
+        bf16 out[NOUT][kTileRows][64][kC];
+    };
 };
 
 // 512 threads: wavefront w computes row y0 + (w >> 1), output channels 32 (w & 1) .. +31, as two
@@ -297,60 +309,118 @@ __global__ void __launch_bounds__(kAllThreads) conv_lds_all_kernel(const bf16* _
             }
         }
     };
+    // the bias of the lane's output channels (forward modes), loaded once: inside the tile loop its
+    // loads were issued after the next tile's prefetch and waited for it (loads complete in order)
+    float bv[4][4];
+    if (MODE != MODE_BWD_DATA) {
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) bv[g][j] = bias[(cb << 5) + (g << 3) + (kh << 2) + j];
+    }
     if (blockIdx.x < ntiles) fetch(blockIdx.x);
     for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         deposit();
         __syncthreads();
-        floatx16 acc[2];
-        acc[0] = floatx16{0};
-        acc[1] = floatx16{0};
-        const bool row_in = ((tile / nseg) % nyt) * kTileRows + wrow < H;  // uniform over the wavefront
-        if (row_in) {
-#pragma unroll
-            for (int t = 0; t < 9; ++t) {
-                int dy = t / 3 - 1, dx = t % 3 - 1;
-                if (MODE == MODE_BWD_DATA) { dy = -dy; dx = -dx; }
-                const int lr = wrow + 1 + dy;
-#pragma unroll
-                for (int kb = 0; kb < CIN / 16; ++kb) {
-                    const int co = kb * 16 + kh * 8;
-                    const bf16x8 a0 = *reinterpret_cast<const bf16x8*>(&sh.in[lr][i + 1 + dx][co]);
-                    const bf16x8 a1 = *reinterpret_cast<const bf16x8*>(&sh.in[lr][i + 33 + dx][co]);
-                    const bf16x8 bw = *reinterpret_cast<const bf16x8*>(&sh.w[t][(cb << 5) + i][co]);
-                    acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bw, a0, acc[0], 0, 0, 0);
-                    acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bw, a1, acc[1], 0, 0, 0);
-                }
-            }
-        }
-        const int nxt = tile + gridDim.x;
-        if (nxt < ntiles) fetch(nxt);  // in flight during the stores below
         const int seg = tile % nseg, yt = (tile / nseg) % nyt, b = tile / (nseg * nyt);
         const int x0 = seg << 6, y = yt * kTileRows + wrow;
-        if (y < H) {
-            // acc[pb][4 g + j]: pixel x0 + 32 pb + i, channel 32 cb + 8 g + 4 kh + j
+        // the input gradient's gelu'(z) operands, loaded before the products (issued after the
+        // prefetch, their wait would include it)
+        bf16x4 zp[2][4];
+        if (MODE == MODE_BWD_DATA && y < H) {
 #pragma unroll
             for (int pb = 0; pb < 2; ++pb) {
                 const size_t pix = ((size_t)b * H + y) * W + x0 + (pb << 5) + i;
 #pragma unroll
-                for (int g = 0; g < 4; ++g) {
-                    const int oc0 = (cb << 5) + (g << 3) + (kh << 2);
-                    const size_t o = pix * kC + oc0;
-                    bf16x4 ov;
-                    if (MODE == MODE_BWD_DATA) {
-                        const bf16x4 zp = *reinterpret_cast<const bf16x4*>(zprev + o);
+                for (int g = 0; g < 4; ++g)
+                    zp[pb][g] = *reinterpret_cast<const bf16x4*>(zprev + pix * kC + (cb << 5) + (g << 3) + (kh << 2));
+            }
+        }
+        floatx16 acc[2];
+        acc[0] = floatx16{0};
+        acc[1] = floatx16{0};
+        const bool row_in = y < H;  // uniform over the wavefront
+        if (row_in) {
+            // products over (tap t, channel block kb), it = t KB + kb, with the three fragments of
+            // step it + PF read from LDS before the MFMAs of step it are issued (a ring of PF + 1
+            // fragment sets; the scheduling barriers keep that order): each step's reads then
+            // have PF steps' MFMAs to arrive in, instead of one LDS latency between steps (PF = 1
+            // at CIN = 32, where two workgroups per CU need at most 128 VGPRs)
+            constexpr int KB = CIN / 16, NIT = 9 * KB, PF = CIN == 64 ? 2 : 1;
+            auto ld = [&](int it, bf16x8& a0, bf16x8& a1, bf16x8& bw) {
+                const int t = it / KB, kb = it % KB;
+                int dy = t / 3 - 1, dx = t % 3 - 1;
+                if (MODE == MODE_BWD_DATA) { dy = -dy; dx = -dx; }
+                const int lr = wrow + 1 + dy, co = kb * 16 + kh * 8;
+                a0 = *reinterpret_cast<const bf16x8*>(&sh.in[lr][i + 1 + dx][co]);
+                a1 = *reinterpret_cast<const bf16x8*>(&sh.in[lr][i + 33 + dx][co]);
+                bw = *reinterpret_cast<const bf16x8*>(&sh.w[t][(cb << 5) + i][co]);
+            };
+            bf16x8 fa0[PF + 1], fa1[PF + 1], fb[PF + 1];
 #pragma unroll
-                        for (int j = 0; j < 4; ++j) ov[j] = (bf16)(acc[pb][4 * g + j] * gelu_grad((float)zp[j]));
-                    } else {
-                        bf16x4 zv;
+            for (int it = 0; it < PF; ++it) ld(it, fa0[it], fa1[it], fb[it]);
 #pragma unroll
-                        for (int j = 0; j < 4; ++j) {
-                            zv[j] = (bf16)(acc[pb][4 * g + j] + bias[oc0 + j]);
-                            if (MODE == MODE_FWD_GELU) ov[j] = (bf16)gelu((float)zv[j]);
+            for (int it = 0; it < NIT; ++it) {
+                if (it + PF < NIT) ld(it + PF, fa0[(it + PF) % (PF + 1)], fa1[(it + PF) % (PF + 1)], fb[(it + PF) % (PF + 1)]);
+                __builtin_amdgcn_sched_barrier(0);
+                const int s = it % (PF + 1);
+                acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fb[s], fa0[s], acc[0], 0, 0, 0);
+                acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fb[s], fa1[s], acc[1], 0, 0, 0);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+        }
+        const int nxt = tile + gridDim.x;
+        if (nxt < ntiles) fetch(nxt);  // in flight during the stores below
+        {
+            // acc[pb][4 g + j]: pixel x0 + 32 pb + i, channel 32 cb + 8 g + 4 kh + j
+            constexpr int NOUT = S::NOUT;
+            const bool has_z = MODE != MODE_BWD_DATA && zout, has_a = MODE != MODE_FWD_LINEAR;
+            // part 0 is z, part 1 is a; NOUT = 2 stages both at once, NOUT = 1 one per round
+            const int rounds = NOUT == 2 ? 1 : (has_z ? 1 : 0) + (has_a ? 1 : 0);
+            const int y0 = yt * kTileRows;
+            for (int rd = 0; rd < rounds; ++rd) {
+                // rd = 0: every wavefront is done with sh.in, whose space the outputs take;
+                // rd = 1: the previous round's stores have read the buffer
+                __syncthreads();
+                if (y < H) {
+#pragma unroll
+                    for (int pb = 0; pb < 2; ++pb)
+#pragma unroll
+                        for (int g = 0; g < 4; ++g) {
+                            bf16x4 zv, ov;
+#pragma unroll
+                            for (int j = 0; j < 4; ++j) {
+                                if (MODE == MODE_BWD_DATA) {
+                                    ov[j] = (bf16)(acc[pb][4 * g + j] * gelu_grad((float)zp[pb][g][j]));
+                                } else {
+                                    zv[j] = (bf16)(acc[pb][4 * g + j] + bv[g][j]);
+                                    if (MODE == MODE_FWD_GELU) ov[j] = (bf16)gelu((float)zv[j]);
+                                }
+                            }
+                            const int px = (pb << 5) + i, c = (cb << 2) + g;
+                            const int l = (((c ^ (px & 7)) << 3) + (kh << 2));
+                            if (NOUT == 2) {
+                                if (has_z) *reinterpret_cast<bf16x4*>(&sh.out[0][wrow][px][l]) = zv;
+                                if (has_a) *reinterpret_cast<bf16x4*>(&sh.out[NOUT - 1][wrow][px][l]) = ov;
+                            } else {
+                                const bool z_now = has_z && rd == 0;
+                                *reinterpret_cast<bf16x4*>(&sh.out[0][wrow][px][l]) = z_now ? zv : ov;
+                            }
                         }
-                        if (zout) *reinterpret_cast<bf16x4*>(zout + o) = zv;
-                        if (MODE != MODE_FWD_GELU) continue;
-                    }
-                    *reinterpret_cast<bf16x4*>(aout + o) = ov;
+                }
+                __syncthreads();
+                const bool st_z = has_z && (NOUT == 2 || rd == 0);
+                const bool st_a = has_a && (NOUT == 2 || rd == rounds - 1);
+                bf16* zdst = st_z ? zout : nullptr;
+                for (int q = tid; q < kTileRows * 64 * 8; q += NT) {
+                    const int r = q >> 9, px = (q >> 3) & 63, c = q & 7;
+                    if (y0 + r >= H) continue;
+                    const size_t o = (((size_t)b * H + y0 + r) * W + x0 + px) * kC + (c << 3);
+                    const int l = (c ^ (px & 7)) << 3;
+                    if (zdst) *reinterpret_cast<bf16x8*>(zdst + o) = *reinterpret_cast<const bf16x8*>(&sh.out[0][r][px][l]);
+                    if (st_a)
+                        *reinterpret_cast<bf16x8*>(aout + o) =
+                            *reinterpret_cast<const bf16x8*>(&sh.out[NOUT == 2 ? 1 : 0][r][px][l]);
                 }
             }
         }
