@@ -209,15 +209,16 @@ __global__ void chunk_boundary_cp_kernel(double* __restrict__ cpb, const double*
     }
 }
 
-// g_r = 1 / (b_r - a_r cp_{r-1}) of one column (the Nyquist system of slot 0), all rows.
-__global__ void column_g_kernel(double* __restrict__ gcol, const double* __restrict__ a_sub,
-                                const double* __restrict__ c_sup, double kk, int Ni) {
+// The pivots g_r of the single system with kx^2 = kk[0] (the Nyquist system), serially, with the
+// operations of the solve kernel's sweep.
+__global__ void column_pivot_kernel(double* __restrict__ g, const double* __restrict__ a_sub,
+                                    const double* __restrict__ c_sup, const double* __restrict__ kk, int Ni) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     double cp_prev = 0.0;
     for (int r = 0; r < Ni; ++r) {
         const double a = a_sub[r], cc = c_sup[r];
-        const double gi = pivot_recip((-(a + cc) - kk) - a * cp_prev);
-        gcol[r] = gi;
+        const double gi = pivot_recip((-(a + cc) - kk[0]) - a * cp_prev);
+        g[r] = gi;
         cp_prev = cc * gi;
     }
 }
@@ -231,24 +232,28 @@ __global__ void column_g_kernel(double* __restrict__ gcol, const double* __restr
 // Per chunk [s, e]: dp_r = dpL_r + P_r D with P_r = prod_{s..r} (-a_i g_i), D = dp_{s-1};
 // x_s = S + RP D + QS X with X = x_{e+1}, S = sum_r W_r dpL_r, RP = sum_r W_r P_r,
 // W_r = prod_{s..r-1} (-cp_i), QS = prod_{s..e} (-cp_i).
-// Real and imaginary parts carry their own coefficient chains; they differ only in slot 0 (real:
-// kx = 0, imaginary: Nyquist), whose second chain reads its precomputed g (gny).
+// Real and imaginary parts share the coefficient chain of their column, except in slot 0 (real:
+// kx = 0, imaginary: Nyquist), whose imaginary part takes the Nyquist pivots g from a table.
 template <int RC, int KC, int T>
 struct ThomasShared {
-    // The row coefficients a_r and c_r of every row, staged in LDS when they fit (up to 2048
-    // rows): the serial loops then read them at LDS latency instead of one dependent L2 round
-    // trip per row (measured 38 -> 25 us per solve at 4096x2049).
+    // The row coefficients a_r and c_r of every row, and the Nyquist system's pivots, staged in
+    // LDS when they fit (up to 2048 rows): the serial loops then read them at LDS latency instead
+    // of one dependent L2 round trip per row (measured 38 -> 25 us per solve at 4096x2049).
     static constexpr bool COEF = (T / KC) * RC <= 2048;
     double sm[2][4][T];
-    // g of the Nyquist chain (slot 0's imaginary part), staged in LDS by the one workgroup that
-    // holds slot 0 so that its serial loops do not wait on a global load per row
-    double gq[(T / KC) * RC];
-    double ca[COEF ? (T / KC) * RC : 1], cc[COEF ? (T / KC) * RC : 1];
+    double ca[COEF ? (T / KC) * RC : 1], cc[COEF ? (T / KC) * RC : 1], gq[COEF ? (T / KC) * RC : 1];
 };
 
-// The solve of column group `grp` (columns grp*KC .. grp*KC+KC-1) of one spectrum. SPLIT: the
-// group holds slot 0, whose imaginary part (Nyquist) has its own coefficient chain; in every
-// other group the imaginary parts share the real parts' chains (P, W, RP, Q), computed once.
+// The solve of column group `grp` (columns grp*KC .. grp*KC+KC-1) of one spectrum. Each lane
+// carries two coefficient chains, x for the real part and y for the imaginary part, which are the
+// same chain (same operations, same values) except in slot 0, the first column of group 0: there
+// the real part is the kx = 0 system, with the wall values, and the imaginary part the Nyquist
+// system, whose pivots g are read from the table gny (computed once per plan by the same
+// recurrence). Every workgroup runs the same code. (Measured at 4096x2049: slot 0 in a code path
+// of its own, in group 0 or in a 257th workgroup, ended 5 to 9 us after the other groups: a path
+// that one CU runs fetches its instructions from memory, and one CU reading and writing the 2047
+// rows of a lone column is slow; a 257th workgroup moreover waits for a CU to free, one
+// 512-lane workgroup filling a CU's registers.)
 //
 // Issue order: the small coefficient loads are staged in LDS first, before any column load is
 // issued (under the flood of every workgroup's column loads they return only after most of the
@@ -258,62 +263,70 @@ struct ThomasShared {
 // for its own value, and the sweep runs while the column arrives instead of after the last load
 // has been issued (issue stalls while the memory queues are full; at 4096x2049 the issue of a
 // whole column took 6.6 us). Measured at 4096x2049: 28.6 -> 24.8 us per solve.
-template <int RC, int KC, int T, bool SPLIT>
-__device__ __forceinline__ void thomas_group(ThomasShared<RC, KC, T>& shm, int grp, cplx* __restrict__ hat, int M, int Ni,
-                                    const double* __restrict__ a_sub, const double* __restrict__ c_sup,
-                                    const double* __restrict__ kx2, const double* __restrict__ cpb,
-                                    const double* __restrict__ gny, double bc_bot, double bc_top) {
+template <int RC, int KC, int T>
+__device__ __forceinline__ void thomas_group(ThomasShared<RC, KC, T>& shm, int grp, cplx* __restrict__ hat, int M,
+                                             int Ni, const double* __restrict__ a_sub,
+                                             const double* __restrict__ c_sup, const double* __restrict__ kx2,
+                                             const double* __restrict__ cpb, const double* __restrict__ gny,
+                                             double bc_bot, double bc_top) {
     constexpr int NCH = T / KC;
     constexpr int NR = NCH * RC;  // rows covered by the chunks (>= Ni)
     constexpr bool COEF = ThomasShared<RC, KC, T>::COEF;
     auto& sm = shm.sm;
-    auto& gq = shm.gq;
     const int t = threadIdx.x, kc = t % KC, c = t / KC;
     const int k = grp * KC + kc;
     const bool colok = k < M;
-    const bool split = SPLIT && (k == 0);
-    const int ldc = M + 1;
+    const bool split = k == 0;  // slot 0: kx = 0 (real part, with the wall values), Nyquist (imaginary)
     const int r0 = c * RC;
     const int nr = colok ? max(0, min(RC, Ni - r0)) : 0;
     const int kl = colok ? k : M - 1;
-    // coefficients: row coefficients (and the Nyquist g) of the rows this lane stages
+    // coefficients: row coefficients of the rows this lane stages
     constexpr int NS = (NR + T - 1) / T;
-    double sa[COEF ? NS : 1], sc[COEF ? NS : 1], sg[SPLIT ? NS : 1];
-    if (COEF || SPLIT) {
+    double sa[COEF ? NS : 1], sc[COEF ? NS : 1], sg[COEF ? NS : 1];
+    if (COEF) {
 #pragma unroll
         for (int q = 0; q < NS; ++q) {
             const int r = min(t + q * T, Ni - 1);
-            if (COEF) { sa[q] = a_sub[r]; sc[q] = c_sup[r]; }
-            if (SPLIT) sg[q] = gny[r];
+            sa[q] = a_sub[r];
+            sc[q] = c_sup[r];
+            sg[q] = gny[r];
         }
     }
     const double kkx = kx2[kl];
-    const double cpx0 = cpb[(size_t)c * ldc + kl];
-    if (COEF || SPLIT) {
+    const double cpx0 = cpb[(size_t)c * (M + 1) + kl];
+    if (COEF) {
 #pragma unroll
         for (int q = 0; q < NS; ++q) {
             const int r = t + q * T;
             // rows Ni..NR-1 receive row Ni-1's values and are never read
             if (NR % T == 0 || r < NR) {
-                if (COEF) { shm.ca[r] = sa[q]; shm.cc[r] = sc[q]; }
-                if (SPLIT) gq[r] = sg[q];
+                shm.ca[r] = sa[q];
+                shm.cc[r] = sc[q];
+                shm.gq[r] = sg[q];
             }
         }
         __syncthreads();  // uniform over the workgroup
     }
     constexpr int LA = RC < 4 ? RC : 4;  // column loads in flight ahead of the sweep
+    // the lane's column as 32-bit byte offsets from the spectrum's base (a scalar base address
+    // and one offset register per access instead of a 64-bit address per row)
+    const char* hb = reinterpret_cast<const char*>(hat);
+    auto col = [&](int row) {
+        return *reinterpret_cast<const cplx*>(hb + ((unsigned)row * (unsigned)M + (unsigned)kl) * 16u);
+    };
     cplx f[RC];
 #pragma unroll
-    for (int i = 0; i < LA; ++i) f[i] = hat[(size_t)min(r0 + i + 1, Ni) * M + kl];
+    for (int i = 0; i < LA; ++i) f[i] = col(min(r0 + i + 1, Ni));
     auto A = [&](int r) { return COEF ? shm.ca[r] : a_sub[r]; };
     auto C = [&](int r) { return COEF ? shm.cc[r] : c_sup[r]; };
-    double cpx = (c > 0 && nr > 0) ? cpx0 : 0.0;
-    double g[RC];
+    auto G = [&](int r) { return COEF ? shm.gq[r] : gny[r]; };
+    const double cpx_in = (c > 0 && nr > 0) ? cpx0 : 0.0;
+    double cpx = cpx_in;
     cplx dpl = {0.0, 0.0}, S = {0.0, 0.0};
-    double Wx = 1.0, Wy = 1.0, Px = 1.0, Py = 1.0, RPx = 0.0, RPy = 0.0;
+    double Wx = 1.0, Px = 1.0, RPx = 0.0, Wy = 1.0, Py = 1.0, RPy = 0.0;
 #pragma unroll
     for (int i = 0; i < RC; ++i) {
-        if (i + LA < RC) f[i + LA] = hat[(size_t)min(r0 + i + LA + 1, Ni) * M + kl];
+        if (i + LA < RC) f[i + LA] = col(min(r0 + i + LA + 1, Ni));
         // keeps the scheduler from hoisting the later rows' loads above this row's elimination
         __builtin_amdgcn_sched_barrier(0);
         // (a branch per row, as in the sweeps below: unguarded, the unrolled sweep holds more
@@ -321,32 +334,31 @@ __device__ __forceinline__ void thomas_group(ThomasShared<RC, KC, T>& shm, int g
         if (i >= nr) continue;
         const int r = r0 + i;
         const double a = A(r), cc = C(r);
-        double rx = -f[i].x;
-        const double ry = -f[i].y;
-        if (split && r == 0) rx -= a * bc_bot;
-        if (split && r == Ni - 1) rx -= cc * bc_top;
+        const double rx = -f[i].x, ry = -f[i].y;
         const double gx = pivot_recip((-(a + cc) - kkx) - a * cpx);
-        const double gy = split ? gq[r] : gx;
+        double gy = gx;
+        if (split) gy = G(r);
         cpx = cc * gx;
         dpl = {(rx - a * dpl.x) * gx, (ry - a * dpl.y) * gy};
         f[i] = dpl;
-        g[i] = gx;
         Px *= -a * gx;
-        if (SPLIT) Py *= -a * gy;
-        S = {S.x + Wx * dpl.x, S.y + (SPLIT ? Wy : Wx) * dpl.y};
+        Py *= -a * gy;
+        S = {S.x + Wx * dpl.x, S.y + Wy * dpl.y};
         RPx += Wx * Px;
-        if (SPLIT) RPy += Wy * Py;
+        RPy += Wy * Py;
         Wx *= -cpx;
-        if (SPLIT) Wy *= -(cc * gy);
+        Wy *= -(cc * gy);
     }
-    if (!SPLIT) { Py = Px; Wy = Wx; RPy = RPx; }
     // Chunk carries. Forward: D_c = dp at the end of chunk c-1, the composition over chunks
-    // 0..c-1 of the maps D -> E + P D (E = the chunk's local dp at its end). Backward: X_c =
-    // x at the start of chunk c+1, the composition over chunks c+1..NCH-1 of X -> h + Q X
-    // (h = S + RP D, Q = W). The imaginary part's P and Q are the real part's outside the
-    // slot-0 group. Chunk c of column kc sits in lane t = c*KC + kc, so the CPW = 64/KC chunks
-    // of one wavefront are composed with lane shuffles (no barrier); the wavefronts' totals
-    // then meet in LDS (one barrier per direction).
+    // 0..c-1 of the maps D -> E + P D (E = the chunk's local dp at its end), applied to the
+    // carry into chunk 0. Backward: X_c = x at the start of chunk c+1, the composition over
+    // chunks c+1..NCH-1 of X -> h + Q X (h = S + RP D, Q = W), applied to the carry out of the
+    // last chunk. The wall values are these two end carries: with dp_{-1} = x_{-1} = psi_bot_hat
+    // and x_Ni = psi_top_hat the first and last rows' recurrences take the wall terms
+    // -a_0 psi_bot_hat and -c_{Ni-1} psi_top_hat of their right-hand sides; both are zero except in
+    // the kx = 0 system. Chunk c of column kc sits in lane t = c*KC + kc, so the CPW = 64/KC
+    // chunks of one wavefront are composed with lane shuffles (no barrier); the wavefronts'
+    // totals then meet in LDS (one barrier per direction).
     constexpr int CPW = 64 / KC, NW = T / 64;
     const int lane = t & 63, w = t >> 6;
     const bool first_in_wave = lane < KC, last_in_wave = lane >= 64 - KC;
@@ -356,13 +368,12 @@ __device__ __forceinline__ void thomas_group(ThomasShared<RC, KC, T>& shm, int g
 #pragma unroll
         for (int d = 1; d < CPW; d <<= 1) {
             const double ux = __shfl_up(ex, d * KC, 64), uy = __shfl_up(ey, d * KC, 64);
-            const double up = __shfl_up(px, d * KC, 64);
-            const double uq = SPLIT ? __shfl_up(py, d * KC, 64) : up;
+            const double upx = __shfl_up(px, d * KC, 64), upy = __shfl_up(py, d * KC, 64);
             if (lane >= d * KC) {
                 ex += px * ux;
                 ey += py * uy;
-                px *= up;
-                py = SPLIT ? py * uq : px;
+                px *= upx;
+                py *= upy;
             }
         }
         if (last_in_wave) {
@@ -373,11 +384,12 @@ __device__ __forceinline__ void thomas_group(ThomasShared<RC, KC, T>& shm, int g
         }
         // exclusive within the wavefront: the inclusive value of the previous chunk
         double dx = __shfl_up(ex, KC, 64), dy = __shfl_up(ey, KC, 64);
-        double qx = __shfl_up(px, KC, 64), qy = SPLIT ? __shfl_up(py, KC, 64) : qx;
+        double qx = __shfl_up(px, KC, 64), qy = __shfl_up(py, KC, 64);
         if (first_in_wave) { dx = 0.0; dy = 0.0; qx = 1.0; qy = 1.0; }
         __syncthreads();
-        // carry into this wavefront: the earlier wavefronts' totals, composed in order
-        double cx = 0.0, cy = 0.0;
+        // carry into this wavefront: the earlier wavefronts' totals, composed in order on the
+        // carry into chunk 0
+        double cx = split ? bc_bot : 0.0, cy = 0.0;
         for (int v = 0; v < w; ++v) {
             cx = agg[(0 * NW + v) * KC + kc] + agg[(2 * NW + v) * KC + kc] * cx;
             cy = agg[(1 * NW + v) * KC + kc] + agg[(3 * NW + v) * KC + kc] * cy;
@@ -391,13 +403,12 @@ __device__ __forceinline__ void thomas_group(ThomasShared<RC, KC, T>& shm, int g
 #pragma unroll
         for (int d = 1; d < CPW; d <<= 1) {
             const double ux = __shfl_down(hx, d * KC, 64), uy = __shfl_down(hy, d * KC, 64);
-            const double uq = __shfl_down(qx, d * KC, 64);
-            const double ur = SPLIT ? __shfl_down(qy, d * KC, 64) : uq;
+            const double uqx = __shfl_down(qx, d * KC, 64), uqy = __shfl_down(qy, d * KC, 64);
             if (lane + d * KC < 64) {
                 hx += qx * ux;
                 hy += qy * uy;
-                qx *= uq;
-                qy = SPLIT ? qy * ur : qx;
+                qx *= uqx;
+                qy *= uqy;
             }
         }
         if (first_in_wave) {
@@ -407,10 +418,10 @@ __device__ __forceinline__ void thomas_group(ThomasShared<RC, KC, T>& shm, int g
             agg[(3 * NW + w) * KC + kc] = qy;
         }
         double xx = __shfl_down(hx, KC, 64), xy = __shfl_down(hy, KC, 64);
-        double rx = __shfl_down(qx, KC, 64), ry = SPLIT ? __shfl_down(qy, KC, 64) : rx;
+        double rx = __shfl_down(qx, KC, 64), ry = __shfl_down(qy, KC, 64);
         if (last_in_wave) { xx = 0.0; xy = 0.0; rx = 1.0; ry = 1.0; }
         __syncthreads();
-        double cx = 0.0, cy = 0.0;
+        double cx = split ? bc_top : 0.0, cy = 0.0;
         for (int v = NW - 1; v > w; --v) {
             cx = agg[(0 * NW + v) * KC + kc] + agg[(2 * NW + v) * KC + kc] * cx;
             cy = agg[(1 * NW + v) * KC + kc] + agg[(3 * NW + v) * KC + kc] * cy;
@@ -418,27 +429,42 @@ __device__ __forceinline__ void thomas_group(ThomasShared<RC, KC, T>& shm, int g
         S = {xx + rx * cx, xy + ry * cy};  // now X of this chunk
     }
     const cplx X = S;
-    // true dp, then back substitution from x_{e+1} = X
+    // true dp, then back substitution from x_{e+1} = X. The pivots g_r depend on the
+    // coefficients only: they are recomputed here (the same operations as in the forward sweep,
+    // so the same values) rather than held in registers through the scans, and kept as
+    // cp_r = c_r g_r, the back substitution's factor (the y chain's differs only in slot 0 and
+    // is recomputed there from the table).
+    double cp[RC];
     double Qx = 1.0, Qy = 1.0;
+    cpx = cpx_in;
 #pragma unroll
     for (int i = 0; i < RC; ++i) {
         if (i < nr) {
             const int r = r0 + i;
-            const double a = A(r);
-            Qx *= -a * g[i];
-            if (SPLIT) Qy *= -a * (split ? gq[r] : g[i]);
-            f[i] = {f[i].x + Qx * D.x, f[i].y + (SPLIT ? Qy : Qx) * D.y};
+            const double a = A(r), cc = C(r);
+            const double gx = pivot_recip((-(a + cc) - kkx) - a * cpx);
+            double gy = gx;
+            if (split) gy = G(r);
+            cpx = cc * gx;
+            cp[i] = cpx;
+            Qx *= -a * gx;
+            Qy *= -a * gy;
+            f[i] = {f[i].x + Qx * D.x, f[i].y + Qy * D.y};
         }
     }
     cplx xn = X;
+    // the rows' store addresses from a base the compiler cannot tie to the loads' row indices
+    // (it kept those live from the first loads to here, two values in scratch)
+    int rbase = r0 + 1;
+    asm volatile("" : "+v"(rbase));
+    cplx* const dst0 = hat + (size_t)rbase * M + k;
 #pragma unroll
     for (int i = RC - 1; i >= 0; --i) {
         if (i < nr) {
-            const int r = r0 + i;
-            const double cc = C(r);
-            const double gy = split ? gq[r] : g[i];
-            xn = {f[i].x - (cc * g[i]) * xn.x, f[i].y - (cc * gy) * xn.y};
-            hat[(size_t)(r + 1) * M + k] = xn;
+            double cpy = cp[i];
+            if (split) cpy = C(r0 + i) * G(r0 + i);
+            xn = {f[i].x - cp[i] * xn.x, f[i].y - cpy * xn.y};
+            dst0[(size_t)i * M] = xn;  // row r0 + i + 1
         }
     }
     if (c == 0 && colok) {
@@ -447,6 +473,7 @@ __device__ __forceinline__ void thomas_group(ThomasShared<RC, KC, T>& shm, int g
     }
 }
 
+// The solve of one spectrum: workgroup b owns the KC columns of group xcd_tile(b).
 template <int RC, int KC, int T>
 __global__ void __launch_bounds__(T) thomas_chunked_kernel(cplx* __restrict__ hat, int M, int Ni,
                                                            const double* __restrict__ a_sub,
@@ -457,15 +484,12 @@ __global__ void __launch_bounds__(T) thomas_chunked_kernel(cplx* __restrict__ ha
                                                            double bc_top) {
     __shared__ ThomasShared<RC, KC, T> S;
     // column groups contiguous per XCD: groups narrower than a 128-byte line share lines in one L2
-    const int grp = xcd_tile(blockIdx.x, gridDim.x);
-    if (grp == 0)
-        thomas_group<RC, KC, T, true>(S, grp, hat, M, Ni, a_sub, c_sup, kx2, cpb, gny, bc_bot, bc_top);
-    else
-        thomas_group<RC, KC, T, false>(S, grp, hat, M, Ni, a_sub, c_sup, kx2, cpb, gny, bc_bot, bc_top);
+    thomas_group<RC, KC, T>(S, xcd_tile(blockIdx.x, gridDim.x), hat, M, Ni, a_sub, c_sup, kx2, cpb, gny, bc_bot,
+                            bc_top);
 }
 
-// A group of equally sized spectra sharing one plan: workgroup b solves column group
-// b % groups of spectrum b / groups.
+// A group of n equally sized spectra sharing one plan: workgroup b solves column group b % groups
+// of spectrum b / groups (b after the XCD mapping).
 template <int RC, int KC, int T>
 __global__ void __launch_bounds__(T) thomas_chunked_group_kernel(cplx* const* __restrict__ hats, int groups,
                                                                  int M, int Ni, const double* __restrict__ a_sub,
@@ -477,11 +501,7 @@ __global__ void __launch_bounds__(T) thomas_chunked_group_kernel(cplx* const* __
     __shared__ ThomasShared<RC, KC, T> S;
     const int b = xcd_tile(blockIdx.x, gridDim.x);
     const int m = b / groups;
-    const int grp = b - m * groups;
-    if (grp == 0)
-        thomas_group<RC, KC, T, true>(S, grp, hats[m], M, Ni, a_sub, c_sup, kx2, cpb, gny, bc_bot, bc_top);
-    else
-        thomas_group<RC, KC, T, false>(S, grp, hats[m], M, Ni, a_sub, c_sup, kx2, cpb, gny, bc_bot, bc_top);
+    thomas_group<RC, KC, T>(S, b - m * groups, hats[m], M, Ni, a_sub, c_sup, kx2, cpb, gny, bc_bot, bc_top);
 }
 
 int ilog2(int n) {
@@ -503,7 +523,7 @@ struct PoissonImpl {
     cplx* d_twp = nullptr;     // per-pass twiddles of the register row FFT
     double* d_kx2 = nullptr;   // [M+1]
     double* d_cpb = nullptr;   // [NCH][M+1] elimination coefficient entering each chunk
-    double* d_gny = nullptr;   // [Ny-2] g of the Nyquist system (slot 0, imaginary part)
+    double* d_gny = nullptr;   // [Ny-2] pivots g of the Nyquist system (slot 0, imaginary part)
     double* d_asub = nullptr;  // [Ny-2]
     double* d_csup = nullptr;  // [Ny-2]
     cplx* d_hat = nullptr;     // [Ny][M]
@@ -680,11 +700,8 @@ int poisson_enable_mapped(PoissonImpl* p, const double* a_node, const double* a_
     chunk_boundary_cp_kernel<<<ceil_div(p->M + 1, 64), 64, 0, s>>>(p->d_cpb, p->d_asub, p->d_csup, p->d_kx2,
                                                                   p->M + 1, Ni, p->rc, p->nch);
     NSE2D_LAUNCH_CHECK();
-    {
-        const double kny = (2.0 * kPi * (double)p->M) / p->Lx;
-        column_g_kernel<<<1, 64, 0, s>>>(p->d_gny, p->d_asub, p->d_csup, kny * kny, Ni);
-        NSE2D_LAUNCH_CHECK();
-    }
+    column_pivot_kernel<<<1, 64, 0, s>>>(p->d_gny, p->d_asub, p->d_csup, p->d_kx2 + p->M, Ni);
+    NSE2D_LAUNCH_CHECK();
     p->mapped = true;
     return 0;
 }
@@ -978,7 +995,8 @@ void spectra1d_destroy(Spectra1DImpl* sp) {
 
 void poisson_destroy(PoissonImpl* p) {
     if (!p) return;
-    (void)hipFree(p->d_tw); (void)hipFree(p->d_twp); (void)hipFree(p->d_kx2); (void)hipFree(p->d_cpb); (void)hipFree(p->d_gny);
+    (void)hipFree(p->d_tw); (void)hipFree(p->d_twp); (void)hipFree(p->d_kx2); (void)hipFree(p->d_cpb);
+    (void)hipFree(p->d_gny);
     (void)hipFree(p->d_asub); (void)hipFree(p->d_csup); (void)hipFree(p->d_hat);
     delete p;
 }
