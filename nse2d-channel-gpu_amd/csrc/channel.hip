@@ -262,23 +262,84 @@ __device__ __forceinline__ double2 stage_pair(const StageArgs& a, int j, int ix,
     return double2{o[0], o[1]};
 }
 
-// ROWS_FIRST: the row factors of all RY rows are loaded at the start (scalar loads into SGPRs
-// when the arguments are kernel arguments; the group kernels, whose arguments come from device
-// memory, would hold them in VGPRs and keep the row-by-row loads).
+// One wavefront's RY rows [jlo, jhi) of a 128-column strip, swept downwards (increasing j) or,
+// UP, upwards. ROWS_FIRST: the row factors of all RY rows are loaded at the start (scalar loads
+// into SGPRs when the arguments are kernel arguments; the group kernels, whose arguments come
+// from device memory, would hold them in VGPRs and keep the row-by-row loads).
+// rk: the row factors of rows jlo + q (ROWS_FIRST); an upward sweep covers a whole slab (jhi =
+// jlo + RY), so that it reads them at fixed indices.
+template <int STAGE, int RY, int PF, bool ROWS_FIRST, bool UP>
+__device__ __forceinline__ void stage_wave2_sweep(const StageArgs& a, int ix, int jlo, int jhi, bool edge, int hx,
+                                                  const RowCoef (&rk)[ROWS_FIRST ? RY : 1]) {
+    const int Nx = a.Nx, Ny = a.Ny;
+    const double inv4dxde = 0.25 * a.inv_dx * a.inv_deta;
+    const double dt = a.dt_ptr ? *a.dt_ptr : a.dt_val;
+    // step q updates row rowq(q); its new neighbour is row rowq(q) + dj
+    constexpr int dj = UP ? -1 : 1;
+    auto rowq = [&](int q) { return UP ? jhi - 1 - q : jlo + q; };
+    auto inrange = [&](int j) { return j >= 0 && j < Ny; };
+    auto row_factors = [&](int q, int j) {
+        if constexpr (ROWS_FIRST) return rk[UP ? RY - 1 - q : q];
+        else return row_coef(a, j);
+    };
+    const int n = jhi - jlo;
+    // Pb: the row behind the sweep (j - dj), Pc: row j, Pf: the row ahead (j + dj)
+    Row4 Pb{}, Pc{}, Pf{}, Wb{}, Wc{}, Wf{};
+    PairRow ring[PF];
+    double2 wring[PF];
+#pragma unroll
+    for (int f = 0; f < PF; ++f) {
+        if (f < RY && inrange(rowq(0) + dj * (1 + f))) load_pair_row(a, rowq(0) + dj * (1 + f), ix, edge, hx, ring[f]);
+        if (STAGE > 1 && f < RY && f < n) wring[f] = *reinterpret_cast<const double2*>(a.wn + idx2(rowq(f), ix, Nx));
+    }
+    if (inrange(rowq(0) - dj)) {
+        PairRow r;
+        load_pair_row(a, rowq(0) - dj, ix, edge, hx, r);
+        expand_pair_row(r, Pb, Wb);
+    }
+    {
+        PairRow r;
+        load_pair_row(a, rowq(0), ix, edge, hx, r);
+        expand_pair_row(r, Pc, Wc);
+    }
+#pragma unroll
+    for (int q = 0; q < RY; ++q) {
+        if (q >= n) break;
+        const int j = rowq(q);
+        const PairRow cur = ring[q % PF];  // row j + dj
+        const double2 wn_c = (STAGE > 1) ? wring[q % PF] : double2{0.0, 0.0};
+        if (q + PF < RY && inrange(j + dj * (1 + PF))) load_pair_row(a, j + dj * (1 + PF), ix, edge, hx, ring[q % PF]);
+        if (STAGE > 1 && q + PF < RY && q + PF < n)
+            wring[q % PF] = *reinterpret_cast<const double2*>(a.wn + idx2(j + dj * PF, ix, Nx));
+        if (inrange(j + dj)) expand_pair_row(cur, Pf, Wf);
+        const double2 o = UP ? stage_pair<STAGE>(a, j, ix, Pf, Pc, Pb, Wf, Wc, Wb, wn_c, dt, inv4dxde,
+                                                 row_factors(q, j), a.rhs_out)
+                             : stage_pair<STAGE>(a, j, ix, Pb, Pc, Pf, Wb, Wc, Wf, wn_c, dt, inv4dxde,
+                                                 row_factors(q, j), a.rhs_out);
+        *reinterpret_cast<double2*>(a.out + idx2(j, ix, Nx)) = o;
+        Pb = Pc; Pc = Pf; Wb = Wc; Wc = Wf;
+    }
+}
+
+// The wavefronts' RY-row slabs alternate in direction: even slabs are swept downwards, odd ones
+// upwards, so the two rows that neighbouring slabs both read are read by both at the same time
+// (both at their first rows or both at their last), while the lines are in L2. Swept in one
+// direction, a slab reads the row it shares with the slab above first and that slab reads it
+// last, a whole slab's time later: at 4096x2049 one read in five of psi and omega_s then came
+// from HBM again (3.5 instead of 3 fields per stage).
 template <int STAGE, int RY, int SYW, int PF, bool ROWS_FIRST = true>
 __device__ __forceinline__ void stage_wave2_body(const StageArgs& a, int tile) {
     const int Nx = a.Nx, Ny = a.Ny;
     const int lane = threadIdx.x;
     const int ntx = Nx >> 7;
     const int x0 = (tile % ntx) << 7, ix = x0 + 2 * lane;
-    // the wavefront's first row (uniform over the wavefront: its row factors are scalar loads)
-    const int jlo = __builtin_amdgcn_readfirstlane(((tile / ntx) * SYW + (int)threadIdx.y) * RY);
+    // the wavefront's slab and first row (uniform over the wavefront: its row factors are scalar loads)
+    const int slab = __builtin_amdgcn_readfirstlane((tile / ntx) * SYW + (int)threadIdx.y);
+    const int jlo = slab * RY;
     if (jlo >= Ny) return;  // uniform over the wavefront
     const int jhi = min(jlo + RY, Ny);
     const bool edge = (lane == 0) || (lane == 63);
     const int hx = (lane == 0) ? (x0 == 0 ? Nx - 1 : x0 - 1) : (x0 + 128 == Nx ? 0 : x0 + 128);
-    const double inv4dxde = 0.25 * a.inv_dx * a.inv_deta;
-    const double dt = a.dt_ptr ? *a.dt_ptr : a.dt_val;
     // The row factors of all RY rows, loaded first. Loaded row by row, after the previous row's
     // store (which the compiler cannot tell apart from them), each would wait behind the row
     // prefetched ahead of it, and every row would pay a full memory latency.
@@ -287,43 +348,11 @@ __device__ __forceinline__ void stage_wave2_body(const StageArgs& a, int tile) {
 #pragma unroll
         for (int q = 0; q < RY; ++q) rk[q] = row_coef(a, min(jlo + q, Ny - 1));
     }
-    auto row_factors = [&](int q, int j) {
-        if constexpr (ROWS_FIRST) return rk[q];
-        else return row_coef(a, j);
-    };
-    Row4 Pm{}, Pc{}, Pp{}, Wm{}, Wc{}, Wp{};
-    PairRow ring[PF];
-    double2 wring[PF];
-#pragma unroll
-    for (int f = 0; f < PF; ++f) {
-        if (f < RY && jlo + 1 + f < Ny) load_pair_row(a, jlo + 1 + f, ix, edge, hx, ring[f]);
-        if (STAGE > 1 && f < RY && jlo + f < jhi)
-            wring[f] = *reinterpret_cast<const double2*>(a.wn + idx2(jlo + f, ix, Nx));
-    }
-    if (jlo > 0) {
-        PairRow r;
-        load_pair_row(a, jlo - 1, ix, edge, hx, r);
-        expand_pair_row(r, Pm, Wm);
-    }
-    {
-        PairRow r;
-        load_pair_row(a, jlo, ix, edge, hx, r);
-        expand_pair_row(r, Pc, Wc);
-    }
-#pragma unroll
-    for (int q = 0; q < RY; ++q) {
-        const int j = jlo + q;
-        if (j >= jhi) break;
-        const PairRow cur = ring[q % PF];  // row j+1
-        const double2 wn_c = (STAGE > 1) ? wring[q % PF] : double2{0.0, 0.0};
-        if (q + PF < RY && j + 1 + PF < Ny) load_pair_row(a, j + 1 + PF, ix, edge, hx, ring[q % PF]);
-        if (STAGE > 1 && q + PF < RY && j + PF < jhi)
-            wring[q % PF] = *reinterpret_cast<const double2*>(a.wn + idx2(j + PF, ix, Nx));
-        if (j + 1 < Ny) expand_pair_row(cur, Pp, Wp);
-        *reinterpret_cast<double2*>(a.out + idx2(j, ix, Nx)) =
-            stage_pair<STAGE>(a, j, ix, Pm, Pc, Pp, Wm, Wc, Wp, wn_c, dt, inv4dxde, row_factors(q, j), a.rhs_out);
-        Pm = Pc; Pc = Pp; Wm = Wc; Wc = Wp;
-    }
+    // a partial last slab is swept downwards
+    if ((slab & 1) && jhi == jlo + RY)
+        stage_wave2_sweep<STAGE, RY, PF, ROWS_FIRST, true>(a, ix, jlo, jhi, edge, hx, rk);
+    else
+        stage_wave2_sweep<STAGE, RY, PF, ROWS_FIRST, false>(a, ix, jlo, jhi, edge, hx, rk);
 }
 
 // The stage and the forward row transform of the Poisson solve in one kernel, one workgroup per
@@ -387,7 +416,8 @@ __global__ void __launch_bounds__(M / R) stage_fft_kernel(StageArgs a, fft::cplx
 
 template <int STAGE, int RY, int SYW, int PF>
 __global__ void __launch_bounds__(64 * SYW) stage_kernel_wave2(StageArgs a) {
-    stage_wave2_body<STAGE, RY, SYW, PF>(a, xcd_tile(blockIdx.x, gridDim.x));
+    // (the row factors of more than 8 rows do not fit the scalar registers: loaded row by row)
+    stage_wave2_body<STAGE, RY, SYW, PF, (RY <= 8)>(a, xcd_tile(blockIdx.x, gridDim.x));
 }
 
 template <int STAGE, int RY, int SYW, int PF>
