@@ -535,13 +535,11 @@ __global__ void __launch_bounds__(kCflThreads) cfl_rows_kernel(const double* __r
 // in registers and the row after next already in flight. x-neighbours come from DPP shifts and the
 // strip's two halo columns. Workgroup b writes its maxima to partial[a * gridDim.x + b].
 constexpr int kCflRows = 16;
-__global__ void __launch_bounds__(kCflThreads) cfl_rows2_kernel(const double* __restrict__ psi, int Nx, int Ny,
-                                                                double inv_dx, double inv_deta,
-                                                                const double* __restrict__ a_node,
-                                                                double* __restrict__ partial) {
-    __shared__ double sm[3 * 16];
+// Tile b of ntiles of one field; partial[a * ntiles + b] receives maximum a.
+__device__ __forceinline__ void cfl_tile_body(const double* __restrict__ psi, int Nx, int Ny, double inv_dx,
+                                              double inv_deta, const double* __restrict__ a_node, int b, int ntiles,
+                                              double* __restrict__ partial, double* sm) {
     const int nsx = Nx / 512;
-    const int b = xcd_tile(blockIdx.x, gridDim.x);
     const int sx = b % nsx, band = b / nsx;
     const int j0 = 1 + band * kCflRows, j1 = min(j0 + kCflRows, Ny - 1);  // rows [j0, j1)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -576,7 +574,48 @@ __global__ void __launch_bounds__(kCflThreads) cfl_rows2_kernel(const double* __
     double m[3] = {m0, m1, m2};
     block_max<3>(m, sm);
     if (threadIdx.x == 0)
-        for (int k = 0; k < 3; ++k) partial[(size_t)k * gridDim.x + b] = m[k];
+        for (int k = 0; k < 3; ++k) partial[(size_t)k * ntiles + b] = m[k];
+}
+
+__global__ void __launch_bounds__(kCflThreads) cfl_rows2_kernel(const double* __restrict__ psi, int Nx, int Ny,
+                                                                double inv_dx, double inv_deta,
+                                                                const double* __restrict__ a_node,
+                                                                double* __restrict__ partial) {
+    __shared__ double sm[3 * 16];
+    cfl_tile_body(psi, Nx, Ny, inv_dx, inv_deta, a_node, xcd_tile(blockIdx.x, gridDim.x), gridDim.x, partial, sm);
+}
+
+// The same tiles for several fields (ensemble members, Nx a multiple of 512) in one launch:
+// member m owns the global tiles [tile_begin, tile_begin + ntiles); the XCD renumbering runs over
+// the global tile index. One workgroup per member then takes the maxima of its tiles.
+struct CflTileJob {
+    const double* psi;
+    const double* a_node;
+    double inv_dx, inv_deta;
+    int Nx, Ny, tile_begin, ntiles;
+    double* partial;  // [3][ntiles]
+    double* out3;     // max|u|, max|v|, max|v| a
+};
+__global__ void __launch_bounds__(kCflThreads) cfl_tiles_group_kernel(const CflTileJob* __restrict__ jobs, int n) {
+    __shared__ double sm[3 * 16];
+    const int b = xcd_tile(blockIdx.x, gridDim.x);
+    int m = 0;
+    while (m + 1 < n && jobs[m + 1].tile_begin <= b) ++m;
+    const CflTileJob& J = jobs[m];
+    cfl_tile_body(J.psi, J.Nx, J.Ny, J.inv_dx, J.inv_deta, J.a_node, b - J.tile_begin, J.ntiles, J.partial, sm);
+}
+__global__ void __launch_bounds__(kCflThreads) cfl_tiles_final_group_kernel(const CflTileJob* __restrict__ jobs) {
+    __shared__ double sm[3 * 16];
+    const CflTileJob& J = jobs[blockIdx.x];
+    double m[3];
+    for (int k = 0; k < 3; ++k) {
+        double v = 0.0;
+        for (int i = threadIdx.x; i < J.ntiles; i += kCflThreads) v = fmax(v, J.partial[(size_t)k * J.ntiles + i]);
+        m[k] = v;
+    }
+    block_max<3>(m, sm);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 3; ++k) J.out3[k] = m[k];
 }
 
 // One workgroup of 256 threads; each pass issues four loads per maximum before reducing them,
@@ -921,6 +960,10 @@ struct EnsembleImpl {
     MaxUVJob* d_jobs = nullptr;    // [n]
     double* d_partial = nullptr;
     int total_rows = 0;
+    // CFL maxima over 512 x kCflRows tiles when every member has Nx % 512 == 0
+    CflTileJob* d_cfl_jobs = nullptr;  // [n]
+    double* d_cfl_partial = nullptr;   // [3][tiles] per member
+    int cfl_tiles = 0;                 // total tiles; 0: the per-row kernels
     int stage_tile = kStageDefault;
     bool all_wide = false;  // every member has Nx % 128 == 0 (two-column stage kernels allowed)
     bool use_graph = true;
@@ -1013,7 +1056,14 @@ void ens_close(EnsembleImpl* e, int field) {
 
 void ens_record_step(EnsembleImpl* e) {
     const int n = (int)e->mem.size();
-    launch_max_uv_group(e->d_jobs, n, e->total_rows, e->s);
+    if (e->cfl_tiles > 0) {
+        cfl_tiles_group_kernel<<<e->cfl_tiles, kCflThreads, 0, e->s>>>(e->d_cfl_jobs, n);
+        NSE2D_LAUNCH_CHECK();
+        cfl_tiles_final_group_kernel<<<n, kCflThreads, 0, e->s>>>(e->d_cfl_jobs);
+        NSE2D_LAUNCH_CHECK();
+    } else {
+        launch_max_uv_group(e->d_jobs, n, e->total_rows, e->s);
+    }
     ens_dt_kernel<<<ceil_div(n, 64), 64, 0, e->s>>>(e->d_rules, e->scal, n);
     NSE2D_LAUNCH_CHECK();
     ens_launch_stage(e, 1);
@@ -1053,7 +1103,7 @@ void ens_destroy(EnsembleImpl* e) {
         poisson_destroy(c.P);
     }
     for (void* q : {(void*)e->scal, (void*)e->d_rules, (void*)e->d_args, (void*)e->d_tile_begin, (void*)e->d_jobs,
-                    (void*)e->d_partial})
+                    (void*)e->d_partial, (void*)e->d_cfl_jobs, (void*)e->d_cfl_partial})
         (void)hipFree(q);
     if (e->own_stream) (void)hipStreamDestroy(e->s);
     delete e;
@@ -1182,6 +1232,27 @@ EnsembleImpl* ens_create(const nse2d_channel_desc* descs, int n, hipStream_t s) 
     NSE2D_HIP_CHECK(hipMemcpyAsync(e->d_rules, rules.data(), sizeof(DtRule) * n, hipMemcpyHostToDevice, e->s));
     NSE2D_HIP_CHECK(hipMemcpyAsync(e->d_args, args.data(), sizeof(StageArgs) * 3 * n, hipMemcpyHostToDevice, e->s));
     NSE2D_HIP_CHECK(hipMemcpyAsync(e->d_jobs, jobs.data(), sizeof(MaxUVJob) * n, hipMemcpyHostToDevice, e->s));
+    {
+        bool tiled = true;
+        for (const EnsMember& M : e->mem) tiled = tiled && M.d.Nx % 512 == 0 && M.d.Ny >= 3;
+        if (tiled) {
+            std::vector<CflTileJob> tj(n);
+            int tiles = 0;
+            for (int m = 0; m < n; ++m) {
+                const EnsMember& M = e->mem[m];
+                tj[m] = CflTileJob{M.psi, M.d.d_a_node, M.inv_dx, M.inv_deta, M.d.Nx, M.d.Ny, tiles,
+                                   (M.d.Nx / 512) * (int)ceil_div(M.d.Ny - 2, kCflRows), nullptr,
+                                   e->scal + (size_t)m * kEnsScal + 2};
+                tiles += tj[m].ntiles;
+            }
+            NSE2D_HIP_CHECK(hipMalloc(&e->d_cfl_partial, sizeof(double) * 3 * (size_t)tiles));
+            for (int m = 0; m < n; ++m) tj[m].partial = e->d_cfl_partial + 3 * (size_t)tj[m].tile_begin;
+            NSE2D_HIP_CHECK(hipMalloc(&e->d_cfl_jobs, sizeof(CflTileJob) * n));
+            NSE2D_HIP_CHECK(hipMemcpyAsync(e->d_cfl_jobs, tj.data(), sizeof(CflTileJob) * n, hipMemcpyHostToDevice,
+                                           e->s));
+            e->cfl_tiles = tiles;
+        }
+    }
     e->all_wide = true;
     for (const EnsMember& M : e->mem) e->all_wide = e->all_wide && (M.d.Nx % 128 == 0);
     {
@@ -1194,6 +1265,8 @@ EnsembleImpl* ens_create(const nse2d_channel_desc* descs, int n, hipStream_t s) 
     // the generic kernel has no group form; the two-column kernels need every Nx % 128 == 0
     if (e->stage_tile == 0 || (stage_code_wide(e->stage_tile) && !e->all_wide)) e->stage_tile = kStageDefault;
     ens_set_tiles(e);
+    // the host tables above are copied asynchronously and go out of scope here
+    NSE2D_HIP_CHECK(hipStreamSynchronize(e->s));
     return e;
 }
 
