@@ -236,7 +236,7 @@ def les_record(r, a):
         x = r["runs"][name]
         w = x.get("window", {})
         runs[name] = {
-            "finite": x["finite"],
+            "finite": x["finite"], "step_budget_exceeded": x.get("step_budget_exceeded", False),
             "re_tau_err_window": w.get("re_tau_rel_err"), "U_err_window": w.get("U_rel_err"),
             "budget_err_window": w.get("budget"), "traj_err_window_end": x.get("traj_err_window_end"),
             "re_tau_err": x.get("re_tau_rel_err"), "U_err": x.get("U_rel_err"), "budget_err": x.get("budget"),
@@ -247,7 +247,7 @@ def les_record(r, a):
     t0 = a.les_spinup
     dns = r["dns_wall_per_time"]
     for x in runs.values():
-        x["speedup"] = dns / x["wall_s_per_unit_time"] if x["wall_s_per_unit_time"] > 0 else None
+        x["speedup"] = dns / x["wall_s_per_unit_time"] if x["wall_s_per_unit_time"] else None
     # the metric's figure: the fastest closure run whose Re_tau and U(y) stay within 2 % of the
     # filtered DNS over the whole comparison window (null when none does)
     ok = [n for n in ("closure_per_step", "closure_mean_free") if runs[n]["within_2pct"] and runs[n]["speedup"]]

@@ -37,6 +37,10 @@ def ic_args(c):
             "--drag", repr(c["drag"]), "--cfl", repr(c["cfl"]), "--cvisc", repr(c["cvisc"])]
 
 
+class StepBudgetExceeded(RuntimeError):
+    """A coarse run needed more steps than its budget (its dt collapsed)."""
+
+
 class Channel:
     """A channel on the device: metrics (host and device), descriptor and handle."""
 
@@ -121,9 +125,17 @@ class Channel:
         dt_v = d.cvisc * 0.5 / (d.nu * (1.0 / dx ** 2 + 1.0 / d.dy_min ** 2)) if d.nu > 0 else 1e30
         return min(d.dt_max, dt_x, dt_y, dt_v)
 
+    # optional step budget of advance_to (a closure that destabilises the run drives dt down)
+    max_steps = None
+    nsteps = 0
+
     def advance_to(self, t, t_target):
-        """Host-dt steps from t to exactly t_target (the last steps clipped as main.cu:466-480)."""
+        """Host-dt steps from t to exactly t_target (the last steps clipped as main.cu:466-480).
+        Raises StepBudgetExceeded once more than max_steps steps were taken since nsteps was reset."""
         while t < t_target - 1e-12 * max(1.0, abs(t_target)):
+            if self.max_steps is not None and self.nsteps >= self.max_steps:
+                raise StepBudgetExceeded(f"{self.nsteps} steps by t = {t:.4f}")
+            self.nsteps += 1
             dt = self.rule_dt()
             rem = t_target - t
             if rem <= dt:
@@ -311,6 +323,7 @@ def run(fine_cfg, coarse_cfg, *, spinup_time, window_time, n_pairs, holdout, tra
         out["dns_window"] = [t0, t1]
         out["dns_steps_window"] = dns_steps
         out["dns_wall_per_time"] = dns_wall / max(t1 - t0, 1e-30)
+        dns_steps_per_time = dns_steps / max(t1 - t0, 1e-30)
         log(f"{n_pairs} pairs over t=[{t0:.3f}, {t1:.3f}], DNS {dns_steps} steps, "
             f"{out['dns_wall_per_time']:.2f} s per unit time, {len(samples)} samples")
         # 3. training
@@ -348,19 +361,31 @@ def run(fine_cfg, coarse_cfg, *, spinup_time, window_time, n_pairs, holdout, tra
             ch.set_omega(samples[0][1], t0)
             t = t0
             errs, U_acc, re_acc, b_acc = [], [], [], []
-            ok = True
-            for (tk, wk, _, _, _, _) in samples[1:]:
-                t = ch.advance_to(t, tk)
-                w = ch.omega()
-                if not np.all(np.isfinite(w)):
-                    ok = False
-                    break
-                errs.append(_interior_rel(w, wk))
-                U_acc.append(ch.mean_u())
-                re_acc.append(ch.re_tau())
-                b_acc.append(budget(ch.field(nh.FIELD_OMEGA), ch.field(nh.FIELD_PSI)))
+            ok, budget_hit = True, False
+            # a coarse run may take as many steps per unit time as the DNS (8x finer) before it
+            # counts as destabilised
+            ch.nsteps, ch.max_steps = 0, int(dns_steps_per_time * (t_end - t0)) + 1000
+            t_log = t0
+            try:
+                for (tk, wk, _, _, _, _) in samples[1:]:
+                    t = ch.advance_to(t, tk)
+                    w = ch.omega()
+                    if not np.all(np.isfinite(w)):
+                        ok = False
+                        break
+                    errs.append(_interior_rel(w, wk))
+                    U_acc.append(ch.mean_u())
+                    re_acc.append(ch.re_tau())
+                    b_acc.append(budget(ch.field(nh.FIELD_OMEGA), ch.field(nh.FIELD_PSI)))
+                    if t >= t_log + 2.0:
+                        t_log = t
+                        log(f"{name}: t = {t:.2f}, {ch.nsteps} steps")
+            except StepBudgetExceeded as ex:
+                ok, budget_hit = False, True
+                log(f"{name}: stopped, dt collapsed ({ex})")
+            ch.max_steps = None
             nw = n_window - 1  # samples of the training window after the start
-            r = {"finite": ok, "traj_err_end": errs[-1] if errs else None,
+            r = {"finite": ok, "step_budget_exceeded": budget_hit, "traj_err_end": errs[-1] if errs else None,
                  "traj_err_mean": float(np.mean(errs)) if errs else None,
                  "traj_err_window_end": errs[nw - 1] if len(errs) >= nw else None}
             if ok and U_acc:
@@ -379,24 +404,30 @@ def run(fine_cfg, coarse_cfg, *, spinup_time, window_time, n_pairs, holdout, tra
                         float(np.mean([s[4] for s in samples[1:][i * n3:(i + 1) * n3 if i < 2 else None]]))) /
                     float(np.mean([s[4] for s in samples[1:][i * n3:(i + 1) * n3 if i < 2 else None]]))
                     for i in range(3)]
-            # timing over the window with the device dt
-            ch.set_omega(samples[0][1], t0)
-            L.nse2d_device_sync()
-            ta = time.perf_counter()
-            steps = 0
-            while ch.time() < t1:
-                ch.step(20)
-                steps += 20
-            L.nse2d_device_sync()
-            wall = time.perf_counter() - ta
-            r.update(timed_steps=steps, wall_per_time=wall / max(ch.time() - t0, 1e-30))
+            # timing over the window with the device dt (a run that destabilised is not timed)
+            if budget_hit:
+                r.update(timed_steps=0, wall_per_time=None)
+            else:
+                ch.set_omega(samples[0][1], t0)
+                L.nse2d_device_sync()
+                ta = time.perf_counter()
+                steps = 0
+                while ch.time() < t1:
+                    ch.step(20)
+                    steps += 20
+                L.nse2d_device_sync()
+                wall = time.perf_counter() - ta
+                r.update(timed_steps=steps, wall_per_time=wall / max(ch.time() - t0, 1e-30))
             res[name] = r
             log(f"{name}: trajectory error {r['traj_err_end']}, Re_tau {r.get('re_tau')} vs filtered DNS "
                 f"{re_filt:.2f} / fine {re_fine:.2f}, U(y) error {r.get('U_rel_err')}, budget errors "
-                f"{r.get('budget')}, {r['wall_per_time']:.3f} s per unit time")
+                f"{r.get('budget')}, {r['wall_per_time']} s per unit time")
         out["runs"] = res
-        out["speedup_vs_dns"] = out["dns_wall_per_time"] / res["closure"]["wall_per_time"]
-        out["speedup_vs_dns_per_step_closure"] = out["dns_wall_per_time"] / res["closure_per_step"]["wall_per_time"]
+        def speedup(n):
+            w = res[n]["wall_per_time"]
+            return out["dns_wall_per_time"] / w if w else None
+        out["speedup_vs_dns"] = speedup("closure")
+        out["speedup_vs_dns_per_step_closure"] = speedup("closure_per_step")
         return out
     finally:
         les.destroy()
