@@ -957,13 +957,10 @@ struct EnsembleImpl {
     StageArgs* d_args = nullptr;   // [3][n]
     int* d_tile_begin = nullptr;   // [n+1]
     int total_tiles = 0;
-    MaxUVJob* d_jobs = nullptr;    // [n]
-    double* d_partial = nullptr;
-    int total_rows = 0;
-    // CFL maxima over 512 x kCflRows tiles when every member has Nx % 512 == 0
+    // CFL maxima over 512 x kCflRows tiles of every member (members have Nx a power of two >= 512)
     CflTileJob* d_cfl_jobs = nullptr;  // [n]
     double* d_cfl_partial = nullptr;   // [3][tiles] per member
-    int cfl_tiles = 0;                 // total tiles; 0: the per-row kernels
+    int cfl_tiles = 0;                 // total tiles
     int stage_tile = kStageDefault;
     bool all_wide = false;  // every member has Nx % 128 == 0 (two-column stage kernels allowed)
     bool use_graph = true;
@@ -1056,14 +1053,10 @@ void ens_close(EnsembleImpl* e, int field) {
 
 void ens_record_step(EnsembleImpl* e) {
     const int n = (int)e->mem.size();
-    if (e->cfl_tiles > 0) {
-        cfl_tiles_group_kernel<<<e->cfl_tiles, kCflThreads, 0, e->s>>>(e->d_cfl_jobs, n);
-        NSE2D_LAUNCH_CHECK();
-        cfl_tiles_final_group_kernel<<<n, kCflThreads, 0, e->s>>>(e->d_cfl_jobs);
-        NSE2D_LAUNCH_CHECK();
-    } else {
-        launch_max_uv_group(e->d_jobs, n, e->total_rows, e->s);
-    }
+    cfl_tiles_group_kernel<<<e->cfl_tiles, kCflThreads, 0, e->s>>>(e->d_cfl_jobs, n);
+    NSE2D_LAUNCH_CHECK();
+    cfl_tiles_final_group_kernel<<<n, kCflThreads, 0, e->s>>>(e->d_cfl_jobs);
+    NSE2D_LAUNCH_CHECK();
     ens_dt_kernel<<<ceil_div(n, 64), 64, 0, e->s>>>(e->d_rules, e->scal, n);
     NSE2D_LAUNCH_CHECK();
     ens_launch_stage(e, 1);
@@ -1102,8 +1095,8 @@ void ens_destroy(EnsembleImpl* e) {
         (void)hipFree(c.d_tab);
         poisson_destroy(c.P);
     }
-    for (void* q : {(void*)e->scal, (void*)e->d_rules, (void*)e->d_args, (void*)e->d_tile_begin, (void*)e->d_jobs,
-                    (void*)e->d_partial, (void*)e->d_cfl_jobs, (void*)e->d_cfl_partial})
+    for (void* q : {(void*)e->scal, (void*)e->d_rules, (void*)e->d_args, (void*)e->d_tile_begin,
+                    (void*)e->d_cfl_jobs, (void*)e->d_cfl_partial})
         (void)hipFree(q);
     if (e->own_stream) (void)hipStreamDestroy(e->s);
     delete e;
@@ -1194,19 +1187,16 @@ EnsembleImpl* ens_create(const nse2d_channel_desc* descs, int n, hipStream_t s) 
     NSE2D_HIP_CHECK(hipMemsetAsync(e->scal, 0, sizeof(double) * kEnsScal * n, e->s));
     std::vector<DtRule> rules(n);
     std::vector<StageArgs> args(3 * (size_t)n);
-    std::vector<MaxUVJob> jobs(n);
-    int rows = 0;
-    for (int m = 0; m < n; ++m) rows += e->mem[m].d.Ny;
-    e->total_rows = rows;
-    NSE2D_HIP_CHECK(hipMalloc(&e->d_partial, sizeof(double) * 3 * (size_t)rows));
-    rows = 0;
+    std::vector<CflTileJob> tj(n);
+    int tiles = 0;
     for (int m = 0; m < n; ++m) {
         const EnsMember& M = e->mem[m];
         const nse2d_channel_desc& d = M.d;
         rules[m] = DtRule{M.dx, M.deta, d.cfl, d.cvisc, d.nu, M.inv_dx2, 1.0 / (d.dy_min * d.dy_min), d.dt_max};
-        jobs[m] = MaxUVJob{M.psi, d.d_a_node, M.inv_dx, M.inv_deta, d.Nx, d.Ny, rows, e->d_partial + 3 * (size_t)rows,
+        tj[m] = CflTileJob{M.psi, d.d_a_node, M.inv_dx, M.inv_deta, d.Nx, d.Ny, tiles,
+                           (d.Nx / 512) * (int)ceil_div(d.Ny - 2, kCflRows), nullptr,
                            e->scal + (size_t)m * kEnsScal + 2};
-        rows += d.Ny;
+        tiles += tj[m].ntiles;
         for (int st = 1; st <= 3; ++st) {
             StageArgs a{};
             a.psi = M.psi;
@@ -1227,32 +1217,14 @@ EnsembleImpl* ens_create(const nse2d_channel_desc* descs, int n, hipStream_t s) 
     }
     NSE2D_HIP_CHECK(hipMalloc(&e->d_rules, sizeof(DtRule) * n));
     NSE2D_HIP_CHECK(hipMalloc(&e->d_args, sizeof(StageArgs) * 3 * n));
-    NSE2D_HIP_CHECK(hipMalloc(&e->d_jobs, sizeof(MaxUVJob) * n));
     NSE2D_HIP_CHECK(hipMalloc(&e->d_tile_begin, sizeof(int) * (n + 1)));
     NSE2D_HIP_CHECK(hipMemcpyAsync(e->d_rules, rules.data(), sizeof(DtRule) * n, hipMemcpyHostToDevice, e->s));
     NSE2D_HIP_CHECK(hipMemcpyAsync(e->d_args, args.data(), sizeof(StageArgs) * 3 * n, hipMemcpyHostToDevice, e->s));
-    NSE2D_HIP_CHECK(hipMemcpyAsync(e->d_jobs, jobs.data(), sizeof(MaxUVJob) * n, hipMemcpyHostToDevice, e->s));
-    {
-        bool tiled = true;
-        for (const EnsMember& M : e->mem) tiled = tiled && M.d.Nx % 512 == 0 && M.d.Ny >= 3;
-        if (tiled) {
-            std::vector<CflTileJob> tj(n);
-            int tiles = 0;
-            for (int m = 0; m < n; ++m) {
-                const EnsMember& M = e->mem[m];
-                tj[m] = CflTileJob{M.psi, M.d.d_a_node, M.inv_dx, M.inv_deta, M.d.Nx, M.d.Ny, tiles,
-                                   (M.d.Nx / 512) * (int)ceil_div(M.d.Ny - 2, kCflRows), nullptr,
-                                   e->scal + (size_t)m * kEnsScal + 2};
-                tiles += tj[m].ntiles;
-            }
-            NSE2D_HIP_CHECK(hipMalloc(&e->d_cfl_partial, sizeof(double) * 3 * (size_t)tiles));
-            for (int m = 0; m < n; ++m) tj[m].partial = e->d_cfl_partial + 3 * (size_t)tj[m].tile_begin;
-            NSE2D_HIP_CHECK(hipMalloc(&e->d_cfl_jobs, sizeof(CflTileJob) * n));
-            NSE2D_HIP_CHECK(hipMemcpyAsync(e->d_cfl_jobs, tj.data(), sizeof(CflTileJob) * n, hipMemcpyHostToDevice,
-                                           e->s));
-            e->cfl_tiles = tiles;
-        }
-    }
+    NSE2D_HIP_CHECK(hipMalloc(&e->d_cfl_partial, sizeof(double) * 3 * (size_t)tiles));
+    for (int m = 0; m < n; ++m) tj[m].partial = e->d_cfl_partial + 3 * (size_t)tj[m].tile_begin;
+    NSE2D_HIP_CHECK(hipMalloc(&e->d_cfl_jobs, sizeof(CflTileJob) * n));
+    NSE2D_HIP_CHECK(hipMemcpyAsync(e->d_cfl_jobs, tj.data(), sizeof(CflTileJob) * n, hipMemcpyHostToDevice, e->s));
+    e->cfl_tiles = tiles;
     e->all_wide = true;
     for (const EnsMember& M : e->mem) e->all_wide = e->all_wide && (M.d.Nx % 128 == 0);
     {

@@ -291,32 +291,6 @@ __global__ void __launch_bounds__(RB) row_max_uv_kernel(const double* __restrict
     row_max_uv_body(psi, Nx, Ny, inv_dx, inv_deta, a_node, j, partial, sm);
 }
 
-// Several fields (ensemble members) in one launch: workgroup b (after the XCD renumbering) takes
-// row b - row_begin of the member whose row range holds it.
-__global__ void __launch_bounds__(RB) row_max_uv_group_kernel(const MaxUVJob* __restrict__ jobs, int n) {
-    __shared__ double sm[3 * 16];
-    const int b = xcd_tile(blockIdx.x, gridDim.x);
-    int m = 0;
-    while (m + 1 < n && jobs[m + 1].row_begin <= b) ++m;
-    const MaxUVJob& J = jobs[m];
-    row_max_uv_body(J.psi, J.Nx, J.Ny, J.inv_dx, J.inv_deta, J.a_node, b - J.row_begin, J.partial, sm);
-}
-
-// One workgroup per member: the maxima over its rows, in row order.
-__global__ void __launch_bounds__(RB) final_max_group_kernel(const MaxUVJob* __restrict__ jobs) {
-    __shared__ double sm[3 * 16];
-    const MaxUVJob& J = jobs[blockIdx.x];
-    double acc[3];
-    for (int a = 0; a < 3; ++a) {
-        double v = 0.0;
-        for (int i = threadIdx.x; i < J.Ny; i += RB) v = fmax(v, J.partial[(size_t)a * J.Ny + i]);
-        acc[a] = v;
-    }
-    block_max<3>(acc, sm);
-    if (threadIdx.x == 0)
-        for (int a = 0; a < 3; ++a) J.out3[a] = acc[a];
-}
-
 __global__ void zero3_kernel(double* a, double* b, double* c) {
     if (threadIdx.x == 0) { *a = 0.0; *b = 0.0; *c = 0.0; }
 }
@@ -449,17 +423,6 @@ __global__ void __launch_bounds__(RB) diag_final_kernel(const double* __restrict
         out[NSE2D_DIAG_VMAXM] = m[2];
     }
 }
-
-}  // namespace
-
-void launch_max_uv_group(const MaxUVJob* d_jobs, int n, int total_rows, hipStream_t s) {
-    row_max_uv_group_kernel<<<total_rows, RB, 0, s>>>(d_jobs, n);
-    NSE2D_LAUNCH_CHECK();
-    final_max_group_kernel<<<n, RB, 0, s>>>(d_jobs);
-    NSE2D_LAUNCH_CHECK();
-}
-
-namespace {
 
 // The two launches of the CFL maxima with the row partials in `partial` (3 x Ny doubles; every
 // row writes its three entries, so nothing needs clearing between calls).

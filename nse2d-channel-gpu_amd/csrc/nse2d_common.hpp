@@ -116,19 +116,6 @@ __device__ __forceinline__ void block_max(double (&v)[NACC], double* sm) {
 // or streams on one device, like the reference's function-static scratch (diag.cu:843-855).
 double* scratch_doubles(size_t n);
 
-// One member of a grouped CFL-maxima launch (ensembles): its psi field and metrics, the first
-// global row index it owns, its per-row partial buffer (3*Ny doubles) and its 3 outputs
-// (max|u|, max|v|, max|v| a).
-struct MaxUVJob {
-    const double* psi;
-    const double* a_node;
-    double inv_dx, inv_deta;
-    int Nx, Ny, row_begin;
-    double* partial;
-    double* out3;
-};
-void launch_max_uv_group(const MaxUVJob* d_jobs, int n, int total_rows, hipStream_t s);
-
 // Runtime wall-type selection: 0 = no-slip Thom closure, 1 = free-slip (omega_wall = 0).
 // The reference selects this at compile time (ops.cu:108, FREE_SLIP_WALLS).
 int wall_type();
