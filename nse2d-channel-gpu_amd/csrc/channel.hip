@@ -425,6 +425,20 @@ __global__ void __launch_bounds__(64 * SYW) stage_kernel_wave(StageArgs a) {
     stage_wave_body<STAGE, RY, SYW, PF>(a, xcd_tile(blockIdx.x, gridDim.x));  // tiles contiguous per XCD
 }
 
+// The member whose range [begin(m), begin(m + 1)) holds the global tile b (begin ascending,
+// begin(0) = 0): a binary search, log2(n) dependent loads before the tile's own loads (a linear
+// walk over 64 members cost up to 63).
+template <class Begin>
+__device__ __forceinline__ int member_of(int b, int n, Begin begin) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (begin(mid) <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 // The stage of several channels (ensemble members) in one launch: args[m] describes member m and
 // its tiles are the global tiles [tile_begin[m], tile_begin[m+1]); the XCD renumbering runs over
 // the global tile index, so each XCD works through one contiguous range of members' rows.
@@ -432,8 +446,7 @@ template <int STAGE, int RY, int SYW, int PF>
 __global__ void __launch_bounds__(64 * SYW) stage_kernel_wave_group(const StageArgs* __restrict__ args,
                                                                     const int* __restrict__ tile_begin, int n) {
     const int b = xcd_tile(blockIdx.x, gridDim.x);
-    int m = 0;
-    while (m + 1 < n && tile_begin[m + 1] <= b) ++m;
+    const int m = member_of(b, n, [&](int k) { return tile_begin[k]; });
     stage_wave_body<STAGE, RY, SYW, PF>(args[m], b - tile_begin[m]);
 }
 
@@ -443,8 +456,7 @@ template <int STAGE, int RY, int SYW, int PF>
 __global__ void __launch_bounds__(64 * SYW) stage_kernel_wave2_group(const StageArgs* __restrict__ args,
                                                                      const int* __restrict__ tile_begin, int n) {
     const int b = xcd_tile(blockIdx.x, gridDim.x);
-    int m = 0;
-    while (m + 1 < n && tile_begin[m + 1] <= b) ++m;
+    const int m = member_of(b, n, [&](int k) { return tile_begin[k]; });
     stage_wave2_body<STAGE, RY, SYW, PF, false>(args[m], b - tile_begin[m]);
 }
 
@@ -599,8 +611,7 @@ struct CflTileJob {
 __global__ void __launch_bounds__(kCflThreads) cfl_tiles_group_kernel(const CflTileJob* __restrict__ jobs, int n) {
     __shared__ double sm[3 * 16];
     const int b = xcd_tile(blockIdx.x, gridDim.x);
-    int m = 0;
-    while (m + 1 < n && jobs[m + 1].tile_begin <= b) ++m;
+    const int m = member_of(b, n, [&](int k) { return jobs[k].tile_begin; });
     const CflTileJob& J = jobs[m];
     cfl_tile_body(J.psi, J.Nx, J.Ny, J.inv_dx, J.inv_deta, J.a_node, b - J.tile_begin, J.ntiles, J.partial, sm);
 }
