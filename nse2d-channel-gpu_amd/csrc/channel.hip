@@ -978,6 +978,10 @@ struct EnsembleImpl {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int cus = 256;  // compute units of the device (grid of the persistent stage kernel)
+    // size classes after the first run their Poisson solves on streams of their own (default
+    // priority), forked from and joined to s by events
+    std::vector<hipStream_t> cls_s;
+    std::vector<hipEvent_t> cls_ev;  // [0]: the fork, [c]: class c done
 };
 
 namespace {
@@ -1051,14 +1055,29 @@ void ens_launch_stage(EnsembleImpl* e, int stage) {
 
 // Poisson solve and wall closure of every member's stage output: field index 1 (omega1),
 // 2 (omega2) or 0 (omega).
+// The Poisson solves of a stage, one per size class. The classes share no buffers; with several
+// classes, class c > 0 runs on stream cls_s[c], which waits for the work already on s and which s
+// waits for in turn, so that the classes' launches may overlap (parallel branches when the step
+// is captured into a graph).
 void ens_close(EnsembleImpl* e, int field) {
-    for (EnsClass& c : e->cls) {
+    const int ncls = (int)e->cls.size();
+    if (ncls > 1) {
+        NSE2D_HIP_CHECK(hipEventRecord(e->cls_ev[0], e->s));
+        for (int k = 1; k < ncls; ++k) NSE2D_HIP_CHECK(hipStreamWaitEvent(e->cls_s[k], e->cls_ev[0], 0));
+    }
+    for (int k = 0; k < ncls; ++k) {
+        EnsClass& c = e->cls[k];
+        hipStream_t s = k == 0 ? e->s : e->cls_s[k];
         const int nc = (int)c.members.size();
         double** tab = c.d_tab;
-        poisson_forward_group(c.P, tab + (size_t)field * nc, c.d_hats, nc, e->s);
-        poisson_tridiag_group(c.P, c.d_hats, nc, c.psi_bot, c.psi_top, e->s);
+        poisson_forward_group(c.P, tab + (size_t)field * nc, c.d_hats, nc, s);
+        poisson_tridiag_group(c.P, c.d_hats, nc, c.psi_bot, c.psi_top, s);
         poisson_inverse_group(c.P, c.d_hats, tab + (size_t)3 * nc, tab + (size_t)field * nc, nc, c.dy_edge,
-                              c.psi_bot, c.psi_top, c.wall, e->s);
+                              c.psi_bot, c.psi_top, c.wall, s);
+    }
+    for (int k = 1; k < ncls; ++k) {
+        NSE2D_HIP_CHECK(hipEventRecord(e->cls_ev[k], e->cls_s[k]));
+        NSE2D_HIP_CHECK(hipStreamWaitEvent(e->s, e->cls_ev[k], 0));
     }
 }
 
@@ -1109,6 +1128,10 @@ void ens_destroy(EnsembleImpl* e) {
     for (void* q : {(void*)e->scal, (void*)e->d_rules, (void*)e->d_args, (void*)e->d_tile_begin,
                     (void*)e->d_cfl_jobs, (void*)e->d_cfl_partial})
         (void)hipFree(q);
+    for (hipStream_t q : e->cls_s)
+        if (q) (void)hipStreamDestroy(q);
+    for (hipEvent_t v : e->cls_ev)
+        if (v) (void)hipEventDestroy(v);
     if (e->own_stream) (void)hipStreamDestroy(e->s);
     delete e;
 }
@@ -1196,6 +1219,14 @@ EnsembleImpl* ens_create(const nse2d_channel_desc* descs, int n, hipStream_t s) 
     }
     NSE2D_HIP_CHECK(hipMalloc(&e->scal, sizeof(double) * kEnsScal * n));
     NSE2D_HIP_CHECK(hipMemsetAsync(e->scal, 0, sizeof(double) * kEnsScal * n, e->s));
+    e->cls_ev.assign(e->cls.size(), nullptr);
+    e->cls_s.assign(e->cls.size(), nullptr);
+    if (e->cls.size() > 1) {
+        for (size_t k = 0; k < e->cls.size(); ++k)
+            NSE2D_HIP_CHECK(hipEventCreateWithFlags(&e->cls_ev[k], hipEventDisableTiming));
+        for (size_t k = 1; k < e->cls.size(); ++k)
+            NSE2D_HIP_CHECK(hipStreamCreateWithFlags(&e->cls_s[k], hipStreamNonBlocking));
+    }
     std::vector<DtRule> rules(n);
     std::vector<StageArgs> args(3 * (size_t)n);
     std::vector<CflTileJob> tj(n);
