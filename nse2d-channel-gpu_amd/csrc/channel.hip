@@ -966,8 +966,11 @@ struct EnsembleImpl {
     double* scal = nullptr;        // [n][kEnsScal]: t, dt, umax, vmax, vmaxm
     DtRule* d_rules = nullptr;     // [n]
     StageArgs* d_args = nullptr;   // [3][n]
-    int* d_tile_begin = nullptr;   // [n+1]
-    int total_tiles = 0;
+    // The members in size-class order: class k owns the slots [cls_off[k], cls_off[k+1]) of d_args
+    // (per stage) and the first tiles d_tile_begin[cls_off[k] + k .. + nk] of its own launches.
+    std::vector<int> cls_off;      // [ncls+1]
+    std::vector<int> cls_tiles;    // [ncls]: tiles of one stage launch of class k
+    int* d_tile_begin = nullptr;   // [n + ncls]
     // CFL maxima over 512 x kCflRows tiles of every member (members have Nx a power of two >= 512)
     CflTileJob* d_cfl_jobs = nullptr;  // [n]
     double* d_cfl_partial = nullptr;   // [3][tiles] per member
@@ -997,45 +1000,57 @@ void ens_set_tiles(EnsembleImpl* e) {
     const int n = (int)e->mem.size();
     const int rows = stage_code_ry(e->stage_tile) * stage_code_sy(e->stage_tile);
     const int strip = stage_code_wide(e->stage_tile) ? 128 : 64;
-    std::vector<int> tb(n + 1, 0);
-    for (int m = 0; m < n; ++m)
-        tb[m + 1] = tb[m] + (e->mem[m].d.Nx / strip) * (int)ceil_div(e->mem[m].d.Ny, rows);
-    e->total_tiles = tb[n];
-    NSE2D_HIP_CHECK(hipMemcpyAsync(e->d_tile_begin, tb.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice, e->s));
+    const int ncls = (int)e->cls.size();
+    std::vector<int> tb;
+    e->cls_tiles.assign(ncls, 0);
+    for (int k = 0; k < ncls; ++k) {
+        int t = 0;
+        tb.push_back(0);
+        for (int m : e->cls[k].members) {
+            t += (e->mem[m].d.Nx / strip) * (int)ceil_div(e->mem[m].d.Ny, rows);
+            tb.push_back(t);
+        }
+        e->cls_tiles[k] = t;
+    }
+    NSE2D_HIP_CHECK(hipMemcpyAsync(e->d_tile_begin, tb.data(), sizeof(int) * (n + ncls), hipMemcpyHostToDevice,
+                                   e->s));
     NSE2D_HIP_CHECK(hipStreamSynchronize(e->s));
 }
 
-void ens_launch_stage(EnsembleImpl* e, int stage) {
-    const int n = (int)e->mem.size();
-    const StageArgs* args = e->d_args + (size_t)(stage - 1) * n;
+// Stage `stage` of the members of size class k, on stream s.
+void ens_launch_stage(EnsembleImpl* e, int stage, int k, hipStream_t s) {
+    const int n = e->cls_off[k + 1] - e->cls_off[k];
+    const StageArgs* args = e->d_args + (size_t)(stage - 1) * e->mem.size() + e->cls_off[k];
+    const int* tile_begin = e->d_tile_begin + e->cls_off[k] + k;
+    const int tiles = e->cls_tiles[k];
 #define NSE2D_ENS_WAVE(RYV, SYV, PFV)                                                                      \
     case RYV * 100 + SYV * 10 + PFV:                                                                       \
         if (stage == 1)                                                                                    \
-            stage_kernel_wave_group<1, RYV, SYV, PFV><<<e->total_tiles, dim3(64, SYV), 0, e->s>>>(args, e->d_tile_begin, n); \
+            stage_kernel_wave_group<1, RYV, SYV, PFV><<<tiles, dim3(64, SYV), 0, s>>>(args, tile_begin, n); \
         else if (stage == 2)                                                                               \
-            stage_kernel_wave_group<2, RYV, SYV, PFV><<<e->total_tiles, dim3(64, SYV), 0, e->s>>>(args, e->d_tile_begin, n); \
+            stage_kernel_wave_group<2, RYV, SYV, PFV><<<tiles, dim3(64, SYV), 0, s>>>(args, tile_begin, n); \
         else                                                                                               \
-            stage_kernel_wave_group<3, RYV, SYV, PFV><<<e->total_tiles, dim3(64, SYV), 0, e->s>>>(args, e->d_tile_begin, n); \
+            stage_kernel_wave_group<3, RYV, SYV, PFV><<<tiles, dim3(64, SYV), 0, s>>>(args, tile_begin, n); \
         break;
 #define NSE2D_ENS_PERSIST(RYV, SYV, PFV)                                                                   \
     case 110000 + RYV * 100 + SYV * 10 + PFV: {                                                            \
-        const int g = std::min(e->total_tiles, e->cus * (16 / SYV));                                       \
+        const int g = std::min(tiles, e->cus * (16 / SYV));                                       \
         if (stage == 1)                                                                                    \
-            stage_kernel_wave2_group_persistent<1, RYV, SYV, PFV><<<g, dim3(64, SYV), 0, e->s>>>(args, e->d_tile_begin, n, e->total_tiles); \
+            stage_kernel_wave2_group_persistent<1, RYV, SYV, PFV><<<g, dim3(64, SYV), 0, s>>>(args, tile_begin, n, tiles); \
         else if (stage == 2)                                                                               \
-            stage_kernel_wave2_group_persistent<2, RYV, SYV, PFV><<<g, dim3(64, SYV), 0, e->s>>>(args, e->d_tile_begin, n, e->total_tiles); \
+            stage_kernel_wave2_group_persistent<2, RYV, SYV, PFV><<<g, dim3(64, SYV), 0, s>>>(args, tile_begin, n, tiles); \
         else                                                                                               \
-            stage_kernel_wave2_group_persistent<3, RYV, SYV, PFV><<<g, dim3(64, SYV), 0, e->s>>>(args, e->d_tile_begin, n, e->total_tiles); \
+            stage_kernel_wave2_group_persistent<3, RYV, SYV, PFV><<<g, dim3(64, SYV), 0, s>>>(args, tile_begin, n, tiles); \
         break;                                                                                             \
     }
 #define NSE2D_ENS_WAVE2(RYV, SYV, PFV)                                                                     \
     case 10000 + RYV * 100 + SYV * 10 + PFV:                                                               \
         if (stage == 1)                                                                                    \
-            stage_kernel_wave2_group<1, RYV, SYV, PFV><<<e->total_tiles, dim3(64, SYV), 0, e->s>>>(args, e->d_tile_begin, n); \
+            stage_kernel_wave2_group<1, RYV, SYV, PFV><<<tiles, dim3(64, SYV), 0, s>>>(args, tile_begin, n); \
         else if (stage == 2)                                                                               \
-            stage_kernel_wave2_group<2, RYV, SYV, PFV><<<e->total_tiles, dim3(64, SYV), 0, e->s>>>(args, e->d_tile_begin, n); \
+            stage_kernel_wave2_group<2, RYV, SYV, PFV><<<tiles, dim3(64, SYV), 0, s>>>(args, tile_begin, n); \
         else                                                                                               \
-            stage_kernel_wave2_group<3, RYV, SYV, PFV><<<e->total_tiles, dim3(64, SYV), 0, e->s>>>(args, e->d_tile_begin, n); \
+            stage_kernel_wave2_group<3, RYV, SYV, PFV><<<tiles, dim3(64, SYV), 0, s>>>(args, tile_begin, n); \
         break;
     switch (e->stage_tile) {
         NSE2D_ENS_WAVE(8, 4, 1) NSE2D_ENS_WAVE(8, 4, 2) NSE2D_ENS_WAVE(8, 4, 3)
@@ -1053,32 +1068,16 @@ void ens_launch_stage(EnsembleImpl* e, int stage) {
     NSE2D_LAUNCH_CHECK();
 }
 
-// Poisson solve and wall closure of every member's stage output: field index 1 (omega1),
-// 2 (omega2) or 0 (omega).
-// The Poisson solves of a stage, one per size class. The classes share no buffers; with several
-// classes, class c > 0 runs on stream cls_s[c], which waits for the work already on s and which s
-// waits for in turn, so that the classes' launches may overlap (parallel branches when the step
-// is captured into a graph).
-void ens_close(EnsembleImpl* e, int field) {
-    const int ncls = (int)e->cls.size();
-    if (ncls > 1) {
-        NSE2D_HIP_CHECK(hipEventRecord(e->cls_ev[0], e->s));
-        for (int k = 1; k < ncls; ++k) NSE2D_HIP_CHECK(hipStreamWaitEvent(e->cls_s[k], e->cls_ev[0], 0));
-    }
-    for (int k = 0; k < ncls; ++k) {
-        EnsClass& c = e->cls[k];
-        hipStream_t s = k == 0 ? e->s : e->cls_s[k];
-        const int nc = (int)c.members.size();
-        double** tab = c.d_tab;
-        poisson_forward_group(c.P, tab + (size_t)field * nc, c.d_hats, nc, s);
-        poisson_tridiag_group(c.P, c.d_hats, nc, c.psi_bot, c.psi_top, s);
-        poisson_inverse_group(c.P, c.d_hats, tab + (size_t)3 * nc, tab + (size_t)field * nc, nc, c.dy_edge,
-                              c.psi_bot, c.psi_top, c.wall, s);
-    }
-    for (int k = 1; k < ncls; ++k) {
-        NSE2D_HIP_CHECK(hipEventRecord(e->cls_ev[k], e->cls_s[k]));
-        NSE2D_HIP_CHECK(hipStreamWaitEvent(e->s, e->cls_ev[k], 0));
-    }
+// Poisson solve and wall closure of the stage output of size class k's members, on stream s:
+// field index 1 (omega1), 2 (omega2) or 0 (omega).
+void ens_close(EnsembleImpl* e, int field, int k, hipStream_t s) {
+    EnsClass& c = e->cls[k];
+    const int nc = (int)c.members.size();
+    double** tab = c.d_tab;
+    poisson_forward_group(c.P, tab + (size_t)field * nc, c.d_hats, nc, s);
+    poisson_tridiag_group(c.P, c.d_hats, nc, c.psi_bot, c.psi_top, s);
+    poisson_inverse_group(c.P, c.d_hats, tab + (size_t)3 * nc, tab + (size_t)field * nc, nc, c.dy_edge, c.psi_bot,
+                          c.psi_top, c.wall, s);
 }
 
 void ens_record_step(EnsembleImpl* e) {
@@ -1089,12 +1088,28 @@ void ens_record_step(EnsembleImpl* e) {
     NSE2D_LAUNCH_CHECK();
     ens_dt_kernel<<<ceil_div(n, 64), 64, 0, e->s>>>(e->d_rules, e->scal, n);
     NSE2D_LAUNCH_CHECK();
-    ens_launch_stage(e, 1);
-    ens_close(e, 1);
-    ens_launch_stage(e, 2);
-    ens_close(e, 2);
-    ens_launch_stage(e, 3);
-    ens_close(e, 0);
+    // The size classes share nothing between the dt and the time advance: each runs its three
+    // stages and solves as a chain of its own, class k > 0 on stream cls_s[k], forked from and
+    // joined to s by events (one graph branch per class when the step is captured), so that the
+    // classes' launches overlap and each one's tail is filled by the others' work.
+    const int ncls = (int)e->cls.size();
+    if (ncls > 1) {
+        NSE2D_HIP_CHECK(hipEventRecord(e->cls_ev[0], e->s));
+        for (int k = 1; k < ncls; ++k) NSE2D_HIP_CHECK(hipStreamWaitEvent(e->cls_s[k], e->cls_ev[0], 0));
+    }
+    for (int k = 0; k < ncls; ++k) {
+        hipStream_t s = k == 0 ? e->s : e->cls_s[k];
+        ens_launch_stage(e, 1, k, s);
+        ens_close(e, 1, k, s);
+        ens_launch_stage(e, 2, k, s);
+        ens_close(e, 2, k, s);
+        ens_launch_stage(e, 3, k, s);
+        ens_close(e, 0, k, s);
+    }
+    for (int k = 1; k < ncls; ++k) {
+        NSE2D_HIP_CHECK(hipEventRecord(e->cls_ev[k], e->cls_s[k]));
+        NSE2D_HIP_CHECK(hipStreamWaitEvent(e->s, e->cls_ev[k], 0));
+    }
     ens_advance_kernel<<<ceil_div(n, 64), 64, 0, e->s>>>(e->scal, n);
     NSE2D_LAUNCH_CHECK();
 }
@@ -1219,6 +1234,13 @@ EnsembleImpl* ens_create(const nse2d_channel_desc* descs, int n, hipStream_t s) 
     }
     NSE2D_HIP_CHECK(hipMalloc(&e->scal, sizeof(double) * kEnsScal * n));
     NSE2D_HIP_CHECK(hipMemsetAsync(e->scal, 0, sizeof(double) * kEnsScal * n, e->s));
+    // the members' slots in size-class order
+    std::vector<int> slot(n);
+    e->cls_off.assign(1, 0);
+    for (const EnsClass& c : e->cls) {
+        for (size_t i = 0; i < c.members.size(); ++i) slot[c.members[i]] = e->cls_off.back() + (int)i;
+        e->cls_off.push_back(e->cls_off.back() + (int)c.members.size());
+    }
     e->cls_ev.assign(e->cls.size(), nullptr);
     e->cls_s.assign(e->cls.size(), nullptr);
     if (e->cls.size() > 1) {
@@ -1254,12 +1276,12 @@ EnsembleImpl* ens_create(const nse2d_channel_desc* descs, int n, hipStream_t s) 
             a.dt_val = 0.0;
             a.mms = 0;
             a.mp = MmsParams{};
-            args[(size_t)(st - 1) * n + m] = a;
+            args[(size_t)(st - 1) * n + slot[m]] = a;
         }
     }
     NSE2D_HIP_CHECK(hipMalloc(&e->d_rules, sizeof(DtRule) * n));
     NSE2D_HIP_CHECK(hipMalloc(&e->d_args, sizeof(StageArgs) * 3 * n));
-    NSE2D_HIP_CHECK(hipMalloc(&e->d_tile_begin, sizeof(int) * (n + 1)));
+    NSE2D_HIP_CHECK(hipMalloc(&e->d_tile_begin, sizeof(int) * (n + e->cls.size())));
     NSE2D_HIP_CHECK(hipMemcpyAsync(e->d_rules, rules.data(), sizeof(DtRule) * n, hipMemcpyHostToDevice, e->s));
     NSE2D_HIP_CHECK(hipMemcpyAsync(e->d_args, args.data(), sizeof(StageArgs) * 3 * n, hipMemcpyHostToDevice, e->s));
     NSE2D_HIP_CHECK(hipMalloc(&e->d_cfl_partial, sizeof(double) * 3 * (size_t)tiles));
