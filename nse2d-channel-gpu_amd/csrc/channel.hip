@@ -687,7 +687,7 @@ struct ChannelImpl {
     PoissonImpl* P = nullptr;
     double inv_dx = 0, inv_deta = 0, inv_dx2 = 0, inv_deta2 = 0, dx = 0, deta = 0;
     // optional timing of kernel classes with events on the launch stream
-    int stage_tile = 10821;  // stage kernel shape (stage_tile_code; 0: generic kernel)
+    int stage_tile = 10411;  // stage kernel shape (stage_tile_code; 0: generic kernel)
     // NSE2D_STAGE_FFT=1: production stages with the two-column kernel run fused with the forward
     // row transform (stage_fft_kernel) where the plan allows it. Off by default: at 4096x2049 the
     // fused kernel moves the fewest bytes (340 MB per launch against 246 + 134 MB for the pair)
@@ -763,14 +763,15 @@ StageArgs stage_args(ChannelImpl* c, const double* ws, const double* wn, double*
 // prefetch depth PF, coded RY*100 + SYW*10 + PF, plus 10000 for the two-column kernels.
 // NSE2D_STAGE_TILE=RYxSYWxPF (or RYxSYW, PF = 2; a leading 'w' for two columns per lane)
 // selects another instantiated shape (used to tune on the device); 0 selects the generic kernel.
-// Default: w8x2x1 (measured at 4096x2049: 150 us per step for the three stages, against 207 for
-// the best one-column shape 8x4x1: the 16-byte accesses reach the copy bandwidth).
+// Default: w4x1x1, one wavefront of 4 rows per workgroup (measured at 4096x2049 with the slabs
+// swept in alternating directions: 132 us per step for the three stages against 139 us for
+// w8x2x1, the earlier default, and 207 us for the best one-column shape 8x4x1).
 // The manufactured-solution forcing (sin/cos per point) always runs in the generic kernel, so the
 // production kernels carry none of its registers.
 constexpr int kStageDefault = 841;        // one column per lane (ensembles, Nx % 128 != 0)
-constexpr int kStageDefaultWide = 10821;  // two columns per lane, 8 rows x 2 wavefronts
+constexpr int kStageDefaultWide = 10411;  // two columns per lane, 4 rows x 1 wavefront
 int stage_tile_code() {
-    int ry = 8, sy = 2, pf = 1, wide = 1;
+    int ry = 4, sy = 1, pf = 1, wide = 1;
     if (const char* e = std::getenv("NSE2D_STAGE_TILE")) {
         if (e[0] == '0') return 0;
         wide = 0;  // one column per lane unless the shape says otherwise
@@ -784,7 +785,8 @@ int stage_tile_code() {
     switch (c) {
         case 841: case 842: case 843: case 1641: case 1642: case 1643: case 1621: case 1622: case 1623:
         case 3211: case 3212: case 3213:
-        case 10841: case 10842: case 10441: case 11621: case 10821: case 110821: case 110841: return c;
+        case 10841: case 10842: case 10441: case 11621: case 10821: case 10411: case 10421: case 10412:
+        case 110821: case 110841: return c;
         default: return kStageDefaultWide;
     }
 }
@@ -812,6 +814,7 @@ void launch_stage_t(ChannelImpl* c, const StageArgs& a) {
         NSE2D_STAGE_WAVE(32, 1, 1) NSE2D_STAGE_WAVE(32, 1, 2) NSE2D_STAGE_WAVE(32, 1, 3)
         NSE2D_STAGE_WAVE2(8, 4, 1) NSE2D_STAGE_WAVE2(8, 4, 2) NSE2D_STAGE_WAVE2(4, 4, 1)
         NSE2D_STAGE_WAVE2(16, 2, 1) NSE2D_STAGE_WAVE2(8, 2, 1)
+        NSE2D_STAGE_WAVE2(4, 1, 1) NSE2D_STAGE_WAVE2(4, 2, 1) NSE2D_STAGE_WAVE2(4, 1, 2)
         default: {
             const unsigned ntiles = ceil_div(c->d.Nx, SX) * ceil_div(c->d.Ny, SY * RY);
             stage_kernel<STAGE, MMS><<<ntiles, dim3(SX, SY), 0, c->s>>>(a);
@@ -1058,7 +1061,7 @@ void ens_launch_stage(EnsembleImpl* e, int stage, int k, hipStream_t s) {
         NSE2D_ENS_WAVE(16, 2, 1) NSE2D_ENS_WAVE(16, 2, 2) NSE2D_ENS_WAVE(16, 2, 3)
         NSE2D_ENS_WAVE(32, 1, 1) NSE2D_ENS_WAVE(32, 1, 2) NSE2D_ENS_WAVE(32, 1, 3)
         NSE2D_ENS_WAVE2(8, 4, 1) NSE2D_ENS_WAVE2(8, 4, 2) NSE2D_ENS_WAVE2(4, 4, 1)
-        NSE2D_ENS_WAVE2(16, 2, 1) NSE2D_ENS_WAVE2(8, 2, 1)
+        NSE2D_ENS_WAVE2(16, 2, 1) NSE2D_ENS_WAVE2(8, 2, 1) NSE2D_ENS_WAVE2(4, 1, 1) NSE2D_ENS_WAVE2(4, 2, 1)
         NSE2D_ENS_PERSIST(8, 2, 1) NSE2D_ENS_PERSIST(8, 4, 1)
         default: std::abort();
     }
@@ -1356,7 +1359,7 @@ std::string ens_autotune(EnsembleImpl* e, int nsteps) {
     std::string rep;
     {
         const int codes[] = {841,   842,   843,   1641,  1642,  1643,  1621, 1622, 1623, 3211, 3212, 3213,
-                             10821, 10841, 10842, 10441, 11621, 110821, 110841};
+                             10821, 10841, 10842, 10441, 11621, 10411, 10421, 110821, 110841};
         int best = e->stage_tile;
         double tbest = 1e300;
         for (int code : codes) {

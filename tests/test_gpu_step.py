@@ -286,7 +286,7 @@ def test_invalid_wall_type_is_rejected():
 
 
 STAGE_TILES = (["0"] + [f"{ry}x{sy}x{pf}" for ry, sy in ((8, 4), (16, 4), (16, 2), (32, 1)) for pf in (1, 2, 3)]
-               + ["w8x4x1", "w8x4x2", "w4x4x1", "w16x2x1", "w8x2x1"])  # two columns per lane (Nx % 128 == 0)
+               + ["w8x4x1", "w8x4x2", "w4x4x1", "w16x2x1", "w8x2x1", "w4x1x1", "w4x2x1", "w4x1x2"])  # two columns per lane (Nx % 128 == 0)
 
 
 @pytest.mark.parametrize("tile", STAGE_TILES)
