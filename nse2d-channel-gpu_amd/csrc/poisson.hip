@@ -250,10 +250,10 @@ struct ThomasShared {
 // the real part is the kx = 0 system, with the wall values, and the imaginary part the Nyquist
 // system, whose pivots g are read from the table gny (computed once per plan by the same
 // recurrence). Every workgroup runs the same code. (Measured at 4096x2049: slot 0 in a code path
-// of its own, in group 0 or in a 257th workgroup, ended 5 to 9 us after the other groups: a path
-// that one CU runs fetches its instructions from memory, and one CU reading and writing the 2047
-// rows of a lone column is slow; a 257th workgroup moreover waits for a CU to free, one
-// 512-lane workgroup filling a CU's registers.)
+// of its own, in group 0 or in a 257th workgroup, ended 5 to 9 us after the other groups. The
+// Nyquist system solved by group 0 after its columns took 9 us, and a second run of the same
+// code right after it 5 us (its instructions then cached); a 257th workgroup moreover waits for a
+// CU to free, one 512-lane workgroup filling a CU's registers.)
 //
 // Issue order: the small coefficient loads are staged in LDS first, before any column load is
 // issued (under the flood of every workgroup's column loads they return only after most of the
