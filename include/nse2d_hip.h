@@ -324,9 +324,13 @@ int nse2d_channel_set_closure(nse2d_channel_t ch, nse2d_closure_fn fn, void* ctx
  *     nse2d_channel_step_auto would advance it alone under the same kernel shapes. Members that
  *     share Nx, Ny, Lx, metric arrays and wall values form a size class and share one Poisson
  *     plan. Requirements: Nx a power of two in [512, 16384], no manufactured forcing. One step is
- *     one grouped launch per kernel class (per size class for the transforms and the y-solve),
- *     captured once into a hipGraph and replayed. All calls run on the ensemble's stream (a new
- *     stream when s is NULL, since the null stream cannot be captured).
+ *     one grouped CFL launch over all members, then per size class one grouped launch per stage
+ *     and per transform and y-solve, captured once into a hipGraph and replayed. All calls run on
+ *     the ensemble's stream (a new stream when s is NULL, since the null stream cannot be
+ *     captured); with several size classes, the classes after the first run their stages and
+ *     solves on streams the ensemble creates (default priority), forked from and joined back to
+ *     the ensemble's stream by events within each step, so work queued on the ensemble's stream
+ *     after a step sees the whole step.
  * ------------------------------------------------------------------------------------------ */
 typedef struct nse2d_ensemble* nse2d_ensemble_t;
 /* n member descriptors (the metric arrays are borrowed); NULL on error (nse2d_last_error). */
