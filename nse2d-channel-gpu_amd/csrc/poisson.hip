@@ -240,8 +240,23 @@ struct ThomasShared {
     // LDS when they fit (up to 2048 rows): the serial loops then read them at LDS latency instead
     // of one dependent L2 round trip per row (measured 38 -> 25 us per solve at 4096x2049).
     static constexpr bool COEF = (T / KC) * RC <= 2048;
-    double sm[2][4][T];
-    double ca[COEF ? (T / KC) * RC : 1], cc[COEF ? (T / KC) * RC : 1], gq[COEF ? (T / KC) * RC : 1];
+    static constexpr int NAGG = 4 * (T / 64) * KC;  // the carry scans' wavefront totals
+    // Row r sits at r + r / RC: the chunks of one wavefront read their rows at the same time,
+    // and without the padding word per chunk those reads (RC doubles apart) fall in one bank.
+    static_assert((RC & (RC - 1)) == 0, "chunk heights are powers of two");
+    static constexpr int LOG2RC = fft::ilog2c(RC);
+    static constexpr int NCOEF = COEF ? (T / KC) * (RC + 1) : 1;
+    __device__ static int pad(int r) { return r + (r >> LOG2RC); }
+    // The pivots g of each lane's first GS rows, kept from the forward sweep for the second pass
+    // (which otherwise recomputes the serial pivot recurrence: 4.2 of 27 us per solve at
+    // 4096x2049). As many rows as fit beside the rest in the CU's 160 KB; chunks of 8 rows or
+    // fewer keep to 80 KB, so that two workgroups fit a CU where their registers allow it.
+    static constexpr int LDS_BUDGET = (RC >= 16 ? 160 : 80) * 1024;
+    static constexpr int FREE_ROWS = (LDS_BUDGET - 8 * (NAGG + 3 * NCOEF)) / (8 * T);
+    static constexpr int GS = FREE_ROWS < RC ? (FREE_ROWS > 0 ? FREE_ROWS : 0) : RC;
+    double agg[NAGG];
+    double ca[NCOEF], cc[NCOEF], gq[NCOEF];
+    double gs[GS > 0 ? GS : 1][T];
 };
 
 // The solve of column group `grp` (columns grp*KC .. grp*KC+KC-1) of one spectrum. Each lane
@@ -272,7 +287,7 @@ __device__ __forceinline__ void thomas_group(ThomasShared<RC, KC, T>& shm, int g
     constexpr int NCH = T / KC;
     constexpr int NR = NCH * RC;  // rows covered by the chunks (>= Ni)
     constexpr bool COEF = ThomasShared<RC, KC, T>::COEF;
-    auto& sm = shm.sm;
+    constexpr int GS = ThomasShared<RC, KC, T>::GS;
     const int t = threadIdx.x, kc = t % KC, c = t / KC;
     const int k = grp * KC + kc;
     const bool colok = k < M;
@@ -300,9 +315,10 @@ __device__ __forceinline__ void thomas_group(ThomasShared<RC, KC, T>& shm, int g
             const int r = t + q * T;
             // rows Ni..NR-1 receive row Ni-1's values and are never read
             if (NR % T == 0 || r < NR) {
-                shm.ca[r] = sa[q];
-                shm.cc[r] = sc[q];
-                shm.gq[r] = sg[q];
+                const int rp = ThomasShared<RC, KC, T>::pad(r);
+                shm.ca[rp] = sa[q];
+                shm.cc[rp] = sc[q];
+                shm.gq[rp] = sg[q];
             }
         }
         __syncthreads();  // uniform over the workgroup
@@ -317,9 +333,10 @@ __device__ __forceinline__ void thomas_group(ThomasShared<RC, KC, T>& shm, int g
     cplx f[RC];
 #pragma unroll
     for (int i = 0; i < LA; ++i) f[i] = col(min(r0 + i + 1, Ni));
-    auto A = [&](int r) { return COEF ? shm.ca[r] : a_sub[r]; };
-    auto C = [&](int r) { return COEF ? shm.cc[r] : c_sup[r]; };
-    auto G = [&](int r) { return COEF ? shm.gq[r] : gny[r]; };
+    using SH = ThomasShared<RC, KC, T>;
+    auto A = [&](int r) { return COEF ? shm.ca[SH::pad(r)] : a_sub[r]; };
+    auto C = [&](int r) { return COEF ? shm.cc[SH::pad(r)] : c_sup[r]; };
+    auto G = [&](int r) { return COEF ? shm.gq[SH::pad(r)] : gny[r]; };
     const double cpx_in = (c > 0 && nr > 0) ? cpx0 : 0.0;
     double cpx = cpx_in;
     cplx dpl = {0.0, 0.0}, S = {0.0, 0.0};
@@ -336,6 +353,7 @@ __device__ __forceinline__ void thomas_group(ThomasShared<RC, KC, T>& shm, int g
         const double a = A(r), cc = C(r);
         const double rx = -f[i].x, ry = -f[i].y;
         const double gx = pivot_recip((-(a + cc) - kkx) - a * cpx);
+        if (i < GS) shm.gs[i][t] = gx;
         double gy = gx;
         if (split) gy = G(r);
         cpx = cc * gx;
@@ -349,6 +367,11 @@ __device__ __forceinline__ void thomas_group(ThomasShared<RC, KC, T>& shm, int g
         Wx *= -cpx;
         Wy *= -(cc * gy);
     }
+    // Every column load has returned (the sweep has used each row). Said explicitly: rows past the
+    // chunk's end keep their loaded values in f, so without this the compiler finds a load
+    // possibly pending on every f[i] it reads later, and before each back-substitution row it
+    // waited for the previous rows' stores as well (vector memory counts complete in order).
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
     // Chunk carries. Forward: D_c = dp at the end of chunk c-1, the composition over chunks
     // 0..c-1 of the maps D -> E + P D (E = the chunk's local dp at its end), applied to the
     // carry into chunk 0. Backward: X_c = x at the start of chunk c+1, the composition over
@@ -362,7 +385,7 @@ __device__ __forceinline__ void thomas_group(ThomasShared<RC, KC, T>& shm, int g
     constexpr int CPW = 64 / KC, NW = T / 64;
     const int lane = t & 63, w = t >> 6;
     const bool first_in_wave = lane < KC, last_in_wave = lane >= 64 - KC;
-    double* agg = &sm[0][0][0];  // [4][NW][KC] wavefront totals (the scans' scratch)
+    double* agg = shm.agg;  // [4][NW][KC] wavefront totals (the scans' scratch)
     {
         double ex = dpl.x, ey = dpl.y, px = Px, py = Py;
 #pragma unroll
@@ -430,21 +453,30 @@ __device__ __forceinline__ void thomas_group(ThomasShared<RC, KC, T>& shm, int g
     }
     const cplx X = S;
     // true dp, then back substitution from x_{e+1} = X. The pivots g_r depend on the
-    // coefficients only: they are recomputed here (the same operations as in the forward sweep,
-    // so the same values) rather than held in registers through the scans, and kept as
+    // coefficients only: the first GS rows' come back from LDS, the rest are recomputed (the same
+    // operations as in the forward sweep, so the same values), and they are kept as
     // cp_r = c_r g_r, the back substitution's factor (the y chain's differs only in slot 0 and
     // is recomputed there from the table).
     double cp[RC];
     double Qx = 1.0, Qy = 1.0;
     cpx = cpx_in;
+    // each row's coefficients and stored pivot are read one row ahead, outside the row's branch
+    // (inside it, every row waited a full LDS latency for its own reads)
+    auto rowc = [&](int i) { return min(r0 + i, Ni - 1); };
+    double an = A(rowc(0)), cn = C(rowc(0)), qn = G(rowc(0)), gn = GS > 0 ? shm.gs[0][t] : 0.0;
 #pragma unroll
     for (int i = 0; i < RC; ++i) {
+        const double a = an, cc = cn, gq = qn, gsv = gn;
+        if (i + 1 < RC) {
+            an = A(rowc(i + 1));
+            cn = C(rowc(i + 1));
+            qn = G(rowc(i + 1));
+            if (i + 1 < GS) gn = shm.gs[i + 1][t];
+        }
         if (i < nr) {
-            const int r = r0 + i;
-            const double a = A(r), cc = C(r);
-            const double gx = pivot_recip((-(a + cc) - kkx) - a * cpx);
+            const double gx = i < GS ? gsv : pivot_recip((-(a + cc) - kkx) - a * cpx);
             double gy = gx;
-            if (split) gy = G(r);
+            if (split) gy = gq;
             cpx = cc * gx;
             cp[i] = cpx;
             Qx *= -a * gx;
