@@ -411,11 +411,19 @@ __device__ __forceinline__ void thomas_group(ThomasShared<RC, KC, T>& shm, int g
         if (first_in_wave) { dx = 0.0; dy = 0.0; qx = 1.0; qy = 1.0; }
         __syncthreads();
         // carry into this wavefront: the earlier wavefronts' totals, composed in order on the
-        // carry into chunk 0
+        // carry into chunk 0. Each total is read one step ahead of its use (reading it in the
+        // step, every step waited a full LDS latency, up to NW - 1 of them in series).
         double cx = split ? bc_bot : 0.0, cy = 0.0;
+        double tn[4];  // wavefront v's totals: E (real, imaginary), P (real, imaginary)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) tn[q] = agg[(q * NW + 0) * KC + kc];
+#pragma unroll 1
         for (int v = 0; v < w; ++v) {
-            cx = agg[(0 * NW + v) * KC + kc] + agg[(2 * NW + v) * KC + kc] * cx;
-            cy = agg[(1 * NW + v) * KC + kc] + agg[(3 * NW + v) * KC + kc] * cy;
+            const double tc[4] = {tn[0], tn[1], tn[2], tn[3]};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) tn[q] = agg[(q * NW + v + 1) * KC + kc];  // v + 1 <= w < NW
+            cx = tc[0] + tc[2] * cx;
+            cy = tc[1] + tc[3] * cy;
         }
         dpl = {dx + qx * cx, dy + qy * cy};  // now D of this chunk
     }
@@ -445,9 +453,16 @@ __device__ __forceinline__ void thomas_group(ThomasShared<RC, KC, T>& shm, int g
         if (last_in_wave) { xx = 0.0; xy = 0.0; rx = 1.0; ry = 1.0; }
         __syncthreads();
         double cx = split ? bc_top : 0.0, cy = 0.0;
+        double tn[4];  // as in the forward composition, read one step ahead
+#pragma unroll
+        for (int q = 0; q < 4; ++q) tn[q] = agg[(q * NW + NW - 1) * KC + kc];
+#pragma unroll 1
         for (int v = NW - 1; v > w; --v) {
-            cx = agg[(0 * NW + v) * KC + kc] + agg[(2 * NW + v) * KC + kc] * cx;
-            cy = agg[(1 * NW + v) * KC + kc] + agg[(3 * NW + v) * KC + kc] * cy;
+            const double tc[4] = {tn[0], tn[1], tn[2], tn[3]};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) tn[q] = agg[(q * NW + v - 1) * KC + kc];  // v - 1 >= w >= 0
+            cx = tc[0] + tc[2] * cx;
+            cy = tc[1] + tc[3] * cy;
         }
         S = {xx + rx * cx, xy + ry * cy};  // now X of this chunk
     }
