@@ -333,10 +333,17 @@ __device__ __forceinline__ void thomas_group(ThomasShared<RC, KC, T>& shm, int g
     cplx f[RC];
 #pragma unroll
     for (int i = 0; i < LA; ++i) f[i] = col(min(r0 + i + 1, Ni));
+    // the coefficients of row r0 + i of the lane's chunk (i < nr), in LDS at pad(r0) + i, so that
+    // the row is an immediate offset; A1, C1, G1 also take rows past the last interior row (the
+    // Q pass reads a row ahead): LDS holds copies of that row there, global memory is clamped
     using SH = ThomasShared<RC, KC, T>;
-    auto A = [&](int r) { return COEF ? shm.ca[SH::pad(r)] : a_sub[r]; };
-    auto C = [&](int r) { return COEF ? shm.cc[SH::pad(r)] : c_sup[r]; };
-    auto G = [&](int r) { return COEF ? shm.gq[SH::pad(r)] : gny[r]; };
+    const int pb = SH::pad(r0);
+    auto A = [&](int i) { return COEF ? shm.ca[pb + i] : a_sub[r0 + i]; };
+    auto C = [&](int i) { return COEF ? shm.cc[pb + i] : c_sup[r0 + i]; };
+    auto G = [&](int i) { return COEF ? shm.gq[pb + i] : gny[r0 + i]; };
+    auto A1 = [&](int i) { return COEF ? shm.ca[pb + i] : a_sub[min(r0 + i, Ni - 1)]; };
+    auto C1 = [&](int i) { return COEF ? shm.cc[pb + i] : c_sup[min(r0 + i, Ni - 1)]; };
+    auto G1 = [&](int i) { return COEF ? shm.gq[pb + i] : gny[min(r0 + i, Ni - 1)]; };
     const double cpx_in = (c > 0 && nr > 0) ? cpx0 : 0.0;
     double cpx = cpx_in;
     cplx dpl = {0.0, 0.0}, S = {0.0, 0.0};
@@ -349,13 +356,12 @@ __device__ __forceinline__ void thomas_group(ThomasShared<RC, KC, T>& shm, int g
         // (a branch per row, as in the sweeps below: unguarded, the unrolled sweep holds more
         // registers and spills at RC = 32)
         if (i >= nr) continue;
-        const int r = r0 + i;
-        const double a = A(r), cc = C(r);
+        const double a = A(i), cc = C(i);
         const double rx = -f[i].x, ry = -f[i].y;
         const double gx = pivot_recip((-(a + cc) - kkx) - a * cpx);
         if (i < GS) shm.gs[i][t] = gx;
         double gy = gx;
-        if (split) gy = G(r);
+        if (split) gy = G(i);
         cpx = cc * gx;
         dpl = {(rx - a * dpl.x) * gx, (ry - a * dpl.y) * gy};
         f[i] = dpl;
@@ -477,15 +483,14 @@ __device__ __forceinline__ void thomas_group(ThomasShared<RC, KC, T>& shm, int g
     cpx = cpx_in;
     // each row's coefficients and stored pivot are read one row ahead, outside the row's branch
     // (inside it, every row waited a full LDS latency for its own reads)
-    auto rowc = [&](int i) { return min(r0 + i, Ni - 1); };
-    double an = A(rowc(0)), cn = C(rowc(0)), qn = G(rowc(0)), gn = GS > 0 ? shm.gs[0][t] : 0.0;
+    double an = A1(0), cn = C1(0), qn = G1(0), gn = GS > 0 ? shm.gs[0][t] : 0.0;
 #pragma unroll
     for (int i = 0; i < RC; ++i) {
         const double a = an, cc = cn, gq = qn, gsv = gn;
         if (i + 1 < RC) {
-            an = A(rowc(i + 1));
-            cn = C(rowc(i + 1));
-            qn = G(rowc(i + 1));
+            an = A1(i + 1);
+            cn = C1(i + 1);
+            qn = G1(i + 1);
             if (i + 1 < GS) gn = shm.gs[i + 1][t];
         }
         if (i < nr) {
@@ -504,14 +509,17 @@ __device__ __forceinline__ void thomas_group(ThomasShared<RC, KC, T>& shm, int g
     // (it kept those live from the first loads to here, two values in scratch)
     int rbase = r0 + 1;
     asm volatile("" : "+v"(rbase));
-    cplx* const dst0 = hat + (size_t)rbase * M + k;
+    // 32-bit byte offsets from the spectrum's base, as for the loads (a 64-bit address per row
+    // took a 64-bit multiply-add per row)
+    char* const hw = reinterpret_cast<char*>(hat);
+    const unsigned off0 = ((unsigned)rbase * (unsigned)M + (unsigned)k) * 16u, rowb = (unsigned)M * 16u;
 #pragma unroll
     for (int i = RC - 1; i >= 0; --i) {
         if (i < nr) {
             double cpy = cp[i];
-            if (split) cpy = C(r0 + i) * G(r0 + i);
+            if (split) cpy = C(i) * G(i);
             xn = {f[i].x - cp[i] * xn.x, f[i].y - cpy * xn.y};
-            dst0[(size_t)i * M] = xn;  // row r0 + i + 1
+            *reinterpret_cast<cplx*>(hw + (off0 + (unsigned)i * rowb)) = xn;  // row r0 + i + 1
         }
     }
     if (c == 0 && colok) {
