@@ -556,32 +556,43 @@ __device__ __forceinline__ void cfl_tile_body(const double* __restrict__ psi, in
     const int j0 = 1 + band * kCflRows, j1 = min(j0 + kCflRows, Ny - 1);  // rows [j0, j1)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int x0 = sx * 512 + wave * 128, ix = x0 + 2 * lane;
-    const bool edge = (lane == 0) || (lane == 63);
     const int hx = (lane == 0) ? (x0 == 0 ? Nx - 1 : x0 - 1) : (x0 + 128 == Nx ? 0 : x0 + 128);
     const double hdy = 0.5 * inv_deta, hdx = 0.5 * inv_dx;
     auto row = [&](int j) { return *reinterpret_cast<const double2*>(psi + idx2(j, ix, Nx)); };
-    auto halo = [&](int j) { return edge ? psi[idx2(j, hx, Nx)] : 0.0; };
-    double2 pm = row(j0 - 1), pc = row(j0), pp = row(j0 + 1);
-    double hc = halo(j0), hp = (j0 + 1 < j1) ? halo(j0 + 1) : 0.0;
+    // the column beside the strip (used by the edge lanes only; every lane loads it, the others
+    // all from one line)
+    auto halo = [&](int j) { return psi[idx2(j, hx, Nx)]; };
+    // rows j0 - 1 .. j1 in rr (row j0 - 1 + i in rr[i]), halos of rows j0 .. j1 - 1 in hh; each
+    // is issued PD rows before the row that first reads it. The loads are unconditional (clamped
+    // at the band's end, values not used) and the loop is unrolled: with the loads under a branch,
+    // or in a rolled loop, the compiler waited for each row as soon as it was issued (loads
+    // complete in order).
+    constexpr int PD = 4;
+    double2 rr[kCflRows + 2];
+    double hh[kCflRows];
+#pragma unroll
+    for (int i = 0; i < PD + 2; ++i) rr[i] = row(min(j0 - 1 + i, j1));
+#pragma unroll
+    for (int i = 0; i < PD; ++i) hh[i] = halo(min(j0 + i, j1 - 1));
     double m0 = 0.0, m1 = 0.0, m2 = 0.0;
-    for (int j = j0; j < j1; ++j) {
-        // the row after next, in flight while this row is reduced
-        double2 pn = pp;
-        double hn = 0.0;
-        if (j + 2 <= j1) { pn = row(j + 2); hn = (j + 2 < j1) ? halo(j + 2) : 0.0; }
-        const double a = a_node[j];
-        // u = a d psi / d eta (rows j -+ 1), v = -d psi / dx (columns x -+ 1), as diag.cu:740-760
-        const double l = wave_shr1(pc.y, hc), r = wave_shl1(pc.x, hc);  // columns x0+2l-1, x0+2l+2
-        const double u0 = fabs(a * ((pp.x - pm.x) * hdy)), u1 = fabs(a * ((pp.y - pm.y) * hdy));
-        const double v0 = fabs(-((pc.y - l) * hdx)), v1 = fabs(-((r - pc.x) * hdx));
-        m0 = fmax(m0, fmax(u0, u1));
-        m1 = fmax(m1, fmax(v0, v1));
-        m2 = fmax(m2, fmax(v0 * a, v1 * a));
-        pm = pc;
-        pc = pp;
-        pp = pn;
-        hc = hp;
-        hp = hn;
+#pragma unroll
+    for (int q = 0; q < kCflRows; ++q) {
+        const int j = j0 + q;
+        if (q + PD + 2 < kCflRows + 2) rr[q + PD + 2] = row(min(j + PD + 1, j1));
+        if (q + PD < kCflRows) hh[q + PD] = halo(min(j + PD, j1 - 1));
+        __builtin_amdgcn_sched_barrier(0);
+        if (j < j1) {  // uniform over the workgroup (the last band may be shorter)
+            const double2 pm = rr[q], pc = rr[q + 1], pp = rr[q + 2];
+            const double hc = hh[q], a = a_node[j];
+            // u = a d psi / d eta (rows j -+ 1), v = -d psi / dx (columns x -+ 1), as diag.cu:740-760
+            const double l = wave_shr1(pc.y, hc), r = wave_shl1(pc.x, hc);  // columns x0+2l-1, x0+2l+2
+            const double u0 = fabs(a * ((pp.x - pm.x) * hdy)), u1 = fabs(a * ((pp.y - pm.y) * hdy));
+            const double v0 = fabs(-((pc.y - l) * hdx)), v1 = fabs(-((r - pc.x) * hdx));
+            m0 = fmax(m0, fmax(u0, u1));
+            m1 = fmax(m1, fmax(v0, v1));
+            m2 = fmax(m2, fmax(v0 * a, v1 * a));
+        }
+        __builtin_amdgcn_sched_barrier(0);
     }
     double m[3] = {m0, m1, m2};
     block_max<3>(m, sm);

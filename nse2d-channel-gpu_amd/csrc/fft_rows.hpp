@@ -252,7 +252,11 @@ __global__ void __launch_bounds__(M / R) rows_forward(const double* __restrict__
 
 // Two rows per workgroup, rows row0 + b and row0 + b + gridDim.x (when < row0 + nrows): the
 // second row's loads are issued before the first row's LDS passes, so that they stream while the
-// first row is transformed and the chip does not alternate between loading and computing.
+// first row is transformed and the chip does not alternate between loading and computing. They
+// are unconditional (the first row again when there is no second): under a branch, the compiler
+// waited for them before the first row's transform, taking the wait that the path without them
+// needs (loads complete in order). The scheduling barriers keep the first row's loads, the second
+// row's loads and the first row's arithmetic in that order.
 template <int M, int R>
 __global__ void __launch_bounds__(M / R) rows_forward2(const double* __restrict__ omega, cplx* __restrict__ hat,
                                                        const cplx* __restrict__ twp, const cplx* __restrict__ tw,
@@ -262,7 +266,9 @@ __global__ void __launch_bounds__(M / R) rows_forward2(const double* __restrict_
     const bool two = rb < nrows;
     cplx va[R], vb[R];
     load_row<M, R>(omega + (size_t)(row0 + ra) * 2 * M, va);
-    if (two) load_row<M, R>(omega + (size_t)(row0 + rb) * 2 * M, vb);
+    __builtin_amdgcn_sched_barrier(0);
+    load_row<M, R>(omega + (size_t)(row0 + (two ? rb : ra)) * 2 * M, vb);
+    __builtin_amdgcn_sched_barrier(0);
     forward_row_from<M, R>(va, hat + (size_t)(row0 + ra) * M, lds, twp, tw);
     if (two) {
         __syncthreads();  // lds is reused
