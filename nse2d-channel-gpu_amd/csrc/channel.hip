@@ -201,14 +201,17 @@ struct PairRow { double2 p, w; double ph, wh; };
 // (x0+2l-1, x0+2l, x0+2l+1, x0+2l+2) of a loaded row
 struct Row4 { double v[4]; };
 
-__device__ __forceinline__ void load_pair_row(const StageArgs& a, int j, int ix, bool edge, int hx, PairRow& r) {
+// Row j's pair of columns (ix, ix+1) of psi and omega_s, and the column hx beside the strip (used
+// by the edge lanes; the others all load one line). No load is conditional: under a branch the
+// compiler waits, at the join, for as many loads as the path without them needs, and since loads
+// complete in order that included the rows prefetched after them.
+__device__ __forceinline__ void load_pair_row(const StageArgs& a, int j, int ix, int hx, PairRow& r) {
     const double* __restrict__ pr = a.psi + (size_t)j * a.Nx;
     const double* __restrict__ wr = a.ws + (size_t)j * a.Nx;
     r.p = *reinterpret_cast<const double2*>(pr + ix);
     r.w = *reinterpret_cast<const double2*>(wr + ix);
-    r.ph = 0.0;
-    r.wh = 0.0;
-    if (edge) { r.ph = pr[hx]; r.wh = wr[hx]; }
+    r.ph = pr[hx];
+    r.wh = wr[hx];
 }
 __device__ __forceinline__ void expand_pair_row(const PairRow& r, Row4& P4, Row4& W4) {
     P4.v[0] = wave_shr1(r.p.y, r.ph); P4.v[1] = r.p.x; P4.v[2] = r.p.y; P4.v[3] = wave_shl1(r.p.x, r.ph);
@@ -269,7 +272,7 @@ __device__ __forceinline__ double2 stage_pair(const StageArgs& a, int j, int ix,
 // rk: the row factors of rows jlo + q (ROWS_FIRST); an upward sweep covers a whole slab (jhi =
 // jlo + RY), so that it reads them at fixed indices.
 template <int STAGE, int RY, int PF, bool ROWS_FIRST, bool UP>
-__device__ __forceinline__ void stage_wave2_sweep(const StageArgs& a, int ix, int jlo, int jhi, bool edge, int hx,
+__device__ __forceinline__ void stage_wave2_sweep(const StageArgs& a, int ix, int jlo, int jhi, int hx,
                                                   const RowCoef (&rk)[ROWS_FIRST ? RY : 1]) {
     const int Nx = a.Nx, Ny = a.Ny;
     const double inv4dxde = 0.25 * a.inv_dx * a.inv_deta;
@@ -277,7 +280,11 @@ __device__ __forceinline__ void stage_wave2_sweep(const StageArgs& a, int ix, in
     // step q updates row rowq(q); its new neighbour is row rowq(q) + dj
     constexpr int dj = UP ? -1 : 1;
     auto rowq = [&](int q) { return UP ? jhi - 1 - q : jlo + q; };
-    auto inrange = [&](int j) { return j >= 0 && j < Ny; };
+    // Every load is unconditional: rows beyond the walls are clamped to the wall row and rows past
+    // a short last slab to its last row (the wall rows take no stencil, and nothing reads the
+    // others). Under a branch, the compiler waited at the join for as many loads as the path
+    // without them needs, which (loads complete in order) included the row prefetched ahead.
+    auto wall = [&](int j) { return min(max(j, 0), Ny - 1); };
     auto row_factors = [&](int q, int j) {
         if constexpr (ROWS_FIRST) return rk[UP ? RY - 1 - q : q];
         else return row_coef(a, j);
@@ -289,17 +296,17 @@ __device__ __forceinline__ void stage_wave2_sweep(const StageArgs& a, int ix, in
     double2 wring[PF];
 #pragma unroll
     for (int f = 0; f < PF; ++f) {
-        if (f < RY && inrange(rowq(0) + dj * (1 + f))) load_pair_row(a, rowq(0) + dj * (1 + f), ix, edge, hx, ring[f]);
-        if (STAGE > 1 && f < RY && f < n) wring[f] = *reinterpret_cast<const double2*>(a.wn + idx2(rowq(f), ix, Nx));
+        if (f < RY) load_pair_row(a, wall(rowq(0) + dj * (1 + f)), ix, hx, ring[f]);
+        if (STAGE > 1 && f < RY) wring[f] = *reinterpret_cast<const double2*>(a.wn + idx2(rowq(min(f, n - 1)), ix, Nx));
     }
-    if (inrange(rowq(0) - dj)) {
+    {
         PairRow r;
-        load_pair_row(a, rowq(0) - dj, ix, edge, hx, r);
+        load_pair_row(a, wall(rowq(0) - dj), ix, hx, r);
         expand_pair_row(r, Pb, Wb);
     }
     {
         PairRow r;
-        load_pair_row(a, rowq(0), ix, edge, hx, r);
+        load_pair_row(a, rowq(0), ix, hx, r);
         expand_pair_row(r, Pc, Wc);
     }
 #pragma unroll
@@ -308,10 +315,10 @@ __device__ __forceinline__ void stage_wave2_sweep(const StageArgs& a, int ix, in
         const int j = rowq(q);
         const PairRow cur = ring[q % PF];  // row j + dj
         const double2 wn_c = (STAGE > 1) ? wring[q % PF] : double2{0.0, 0.0};
-        if (q + PF < RY && inrange(j + dj * (1 + PF))) load_pair_row(a, j + dj * (1 + PF), ix, edge, hx, ring[q % PF]);
-        if (STAGE > 1 && q + PF < RY && q + PF < n)
-            wring[q % PF] = *reinterpret_cast<const double2*>(a.wn + idx2(j + dj * PF, ix, Nx));
-        if (inrange(j + dj)) expand_pair_row(cur, Pf, Wf);
+        if (q + PF < RY) load_pair_row(a, wall(j + dj * (1 + PF)), ix, hx, ring[q % PF]);
+        if (STAGE > 1 && q + PF < RY)
+            wring[q % PF] = *reinterpret_cast<const double2*>(a.wn + idx2(rowq(min(q + PF, n - 1)), ix, Nx));
+        expand_pair_row(cur, Pf, Wf);
         const double2 o = UP ? stage_pair<STAGE>(a, j, ix, Pf, Pc, Pb, Wf, Wc, Wb, wn_c, dt, inv4dxde,
                                                  row_factors(q, j), a.rhs_out)
                              : stage_pair<STAGE>(a, j, ix, Pb, Pc, Pf, Wb, Wc, Wf, wn_c, dt, inv4dxde,
@@ -338,7 +345,6 @@ __device__ __forceinline__ void stage_wave2_body(const StageArgs& a, int tile) {
     const int jlo = slab * RY;
     if (jlo >= Ny) return;  // uniform over the wavefront
     const int jhi = min(jlo + RY, Ny);
-    const bool edge = (lane == 0) || (lane == 63);
     const int hx = (lane == 0) ? (x0 == 0 ? Nx - 1 : x0 - 1) : (x0 + 128 == Nx ? 0 : x0 + 128);
     // The row factors of all RY rows, loaded first. Loaded row by row, after the previous row's
     // store (which the compiler cannot tell apart from them), each would wait behind the row
@@ -350,9 +356,9 @@ __device__ __forceinline__ void stage_wave2_body(const StageArgs& a, int tile) {
     }
     // a partial last slab is swept downwards
     if ((slab & 1) && jhi == jlo + RY)
-        stage_wave2_sweep<STAGE, RY, PF, ROWS_FIRST, true>(a, ix, jlo, jhi, edge, hx, rk);
+        stage_wave2_sweep<STAGE, RY, PF, ROWS_FIRST, true>(a, ix, jlo, jhi, hx, rk);
     else
-        stage_wave2_sweep<STAGE, RY, PF, ROWS_FIRST, false>(a, ix, jlo, jhi, edge, hx, rk);
+        stage_wave2_sweep<STAGE, RY, PF, ROWS_FIRST, false>(a, ix, jlo, jhi, hx, rk);
 }
 
 // The stage and the forward row transform of the Poisson solve in one kernel, one workgroup per
@@ -373,7 +379,6 @@ __global__ void __launch_bounds__(M / R) stage_fft_kernel(StageArgs a, fft::cplx
     const int Nx = 2 * M, Ny = a.Ny;
     const int j = xcd_tile(blockIdx.x, gridDim.x);
     const int tid = threadIdx.x, lane = tid & 63;
-    const bool edge = (lane == 0) || (lane == 63);
     const bool interior = j > 0 && j < Ny - 1;  // uniform over the workgroup
     const double inv4dxde = 0.25 * a.inv_dx * a.inv_deta;
     const double dt = a.dt_ptr ? *a.dt_ptr : a.dt_val;
@@ -385,10 +390,10 @@ __global__ void __launch_bounds__(M / R) stage_fft_kernel(StageArgs a, fft::cplx
     struct Slot { PairRow m, c, p; double2 wn; };
     auto load_slot = [&](int q, Slot& S) {
         const int x0 = strip(q), ix = x0 + 2 * lane, hx = halo(x0);
-        load_pair_row(a, j, ix, edge, hx, S.c);
+        load_pair_row(a, j, ix, hx, S.c);
         if (interior) {
-            load_pair_row(a, j - 1, ix, edge, hx, S.m);
-            load_pair_row(a, j + 1, ix, edge, hx, S.p);
+            load_pair_row(a, j - 1, ix, hx, S.m);
+            load_pair_row(a, j + 1, ix, hx, S.p);
         }
         S.wn = (STAGE > 1) ? *reinterpret_cast<const double2*>(a.wn + idx2(j, ix, Nx)) : double2{0.0, 0.0};
     };
