@@ -232,6 +232,14 @@ __device__ __forceinline__ RowCoef row_coef(const StageArgs& a, int j) {
     return {interior ? a.a_node[j] : 0.0, interior ? a.a_edge[j] : 0.0, interior ? a.a_edge[j - 1] : 0.0,
             a.f_row[j]};
 }
+// The same with unconditional loads (indices clamped into the arrays, wall rows then select
+// zero), for the sweeps that load each row's factors with the rows' vector loads: a load under a
+// branch makes the compiler wait, at the join, for the loads issued after it too.
+__device__ __forceinline__ RowCoef row_coef_loaded(const StageArgs& a, int j) {
+    const bool interior = (j > 0 && j < a.Ny - 1);
+    const double an = a.a_node[j], ae = a.a_edge[min(j, a.Ny - 2)], aw = a.a_edge[max(j - 1, 0)];
+    return {interior ? an : 0.0, interior ? ae : 0.0, interior ? aw : 0.0, a.f_row[j]};
+}
 
 // The two updated values (columns ix, ix+1) of row j from the expanded rows j-1, j, j+1 of psi
 // and omega_s and the lane's pair of omega^n (stages 2, 3); stores the RHS copy to rhs_out when
@@ -285,9 +293,11 @@ __device__ __forceinline__ void stage_wave2_sweep(const StageArgs& a, int ix, in
     // others). Under a branch, the compiler waited at the join for as many loads as the path
     // without them needs, which (loads complete in order) included the row prefetched ahead.
     auto wall = [&](int j) { return min(max(j, 0), Ny - 1); };
+    // without ROWS_FIRST, each row's factors are loaded a row ahead, with the row's prefetch
+    RowCoef rkn = ROWS_FIRST ? RowCoef{} : row_coef_loaded(a, rowq(0));
     auto row_factors = [&](int q, int j) {
         if constexpr (ROWS_FIRST) return rk[UP ? RY - 1 - q : q];
-        else return row_coef(a, j);
+        else return rkn;
     };
     const int n = jhi - jlo;
     // Pb: the row behind the sweep (j - dj), Pc: row j, Pf: the row ahead (j + dj)
@@ -315,14 +325,16 @@ __device__ __forceinline__ void stage_wave2_sweep(const StageArgs& a, int ix, in
         const int j = rowq(q);
         const PairRow cur = ring[q % PF];  // row j + dj
         const double2 wn_c = (STAGE > 1) ? wring[q % PF] : double2{0.0, 0.0};
+        const RowCoef rq = row_factors(q, j);
         if (q + PF < RY) load_pair_row(a, wall(j + dj * (1 + PF)), ix, hx, ring[q % PF]);
         if (STAGE > 1 && q + PF < RY)
             wring[q % PF] = *reinterpret_cast<const double2*>(a.wn + idx2(rowq(min(q + PF, n - 1)), ix, Nx));
+        if (!ROWS_FIRST && q + 1 < RY) rkn = row_coef_loaded(a, rowq(min(q + 1, n - 1)));
         expand_pair_row(cur, Pf, Wf);
-        const double2 o = UP ? stage_pair<STAGE>(a, j, ix, Pf, Pc, Pb, Wf, Wc, Wb, wn_c, dt, inv4dxde,
-                                                 row_factors(q, j), a.rhs_out)
-                             : stage_pair<STAGE>(a, j, ix, Pb, Pc, Pf, Wb, Wc, Wf, wn_c, dt, inv4dxde,
-                                                 row_factors(q, j), a.rhs_out);
+        const double2 o = UP ? stage_pair<STAGE>(a, j, ix, Pf, Pc, Pb, Wf, Wc, Wb, wn_c, dt, inv4dxde, rq,
+                                                 a.rhs_out)
+                             : stage_pair<STAGE>(a, j, ix, Pb, Pc, Pf, Wb, Wc, Wf, wn_c, dt, inv4dxde, rq,
+                                                 a.rhs_out);
         *reinterpret_cast<double2*>(a.out + idx2(j, ix, Nx)) = o;
         Pb = Pc; Pc = Pf; Wb = Wc; Wc = Wf;
     }
